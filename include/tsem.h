@@ -1,0 +1,164 @@
+/*
+ * C ABI of the MI355X TSEM operator and solver library (libtsem_amd.so).
+ *
+ * Every entry point returns a status code (TSEM_OK on success); on failure
+ * tsem_last_error() describes the cause.  Vectors marked "device" are HIP
+ * device pointers of length n_dof (double or float as stated); all work runs
+ * on the context's stream and the calls return after it completed unless
+ * stated otherwise.  One context per host thread; calls on one context are
+ * serialized.
+ *
+ * Each function names the TSEMLib interface it replaces (file:line in the
+ * reference tree).
+ */
+#ifndef TSEM_AMD_H
+#define TSEM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TSEM_OK = 0,
+  TSEM_ENOCONV = 1, /* iteration limit reached before the tolerance */
+  TSEM_EINVAL = 2,  /* invalid argument */
+  TSEM_ENOMEM = 3,  /* device or host allocation failed */
+  TSEM_EDEVICE = 4, /* HIP runtime error */
+  TSEM_ESTATE = 5   /* call out of order (e.g. solve before imposing BCs) */
+};
+
+typedef struct tsem_ctx tsem_ctx;
+typedef struct tsem_space tsem_space;
+
+const char* tsem_last_error(void);
+
+/* Context: device + stream.  No counterpart (TSEMLib is host-only). */
+int tsem_ctx_create(int device, tsem_ctx** out);
+int tsem_ctx_destroy(tsem_ctx* ctx);
+
+/* Space of polynomial order M on a tetrahedral mesh; builds the index maps,
+ * reference matrices, geometric factors and the assembled diagonal.
+ * Replaces TSEM<double>::TSEM(M, mesh, fem_space) + init
+ * (include/TetrahedralSEM.h:193-215, src/Initialize.cpp:11-63).
+ * tet: nt x 4 vertex ids (element vertex order P0..P3); femdof_weight:
+ * optional (nv + n_edges) ordering weights of vertices then edges (the P2 fem
+ * dof numbers AFEPack supplies), NULL = the canonical element-scan order;
+ * qorder: Gauss-Legendre points per direction of the collapsed quadrature for
+ * the load vector and boundary data (TSEM(M, qinfo_order, ...),
+ * src/QuadInfo.cpp:63-119). */
+int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, const int32_t* tet,
+                      const int32_t* femdof_weight, int qorder, tsem_space** out);
+int tsem_space_destroy(tsem_space* sp);
+/* ctx may be NULL: a host-only space (setup, maps and reference data only;
+ * device entry points then return TSEM_ESTATE). */
+
+/* Assembled diagonal of the stiffness operator, host copy (n_dof). */
+int tsem_space_diagonal_host(const tsem_space* sp, double* diag);
+
+/* info[0..11] = n_dof, n_index(K), nv, ne, nf, nt, n_dof_geometry[0..3],
+ * n_map_terms, n_qp3 */
+int tsem_space_info(const tsem_space* sp, int64_t* info);
+
+/* Derived topology: edge_vtx[ne][2], face_vtx[nf][3], face_edge[nf][3] (edge
+ * opposite face vertex j), tet_face[nt][4] (face opposite vertex k). */
+int tsem_space_topology(const tsem_space* sp, int32_t* edge_vtx, int32_t* face_vtx, int32_t* face_edge,
+                        int32_t* tet_face);
+
+/* Gather maps (host arrays): local coefficient (e, i) = sum_{t in
+ * [off[eK+i], off[eK+i+1])} val[t] * x[ind[t]].  Replaces the public
+ * transform_{n,ind,val}_global2local (include/TetrahedralSEM.h:97-99). */
+int tsem_space_maps(const tsem_space* sp, int32_t* off, int32_t* ind, double* val);
+
+/* First DOF of every geometry (vertices, edges, faces, tets in that order):
+ * location_actualdof[location_geometry[.]] (src/BuildSpace.cpp:940-945). */
+int tsem_space_dof_start(const tsem_space* sp, int64_t* start);
+
+/* Polynomial degree of each DOF: TSEM::calc_sum_dof (src/Miscellaneous.cpp:557). */
+int tsem_space_sum_dof(const tsem_space* sp, int32_t* sum_dof);
+
+/* Reference matrices (host, K x K row-major): mass, 6 pair stiffness matrices
+ * (stiff_matrix_element_actual, src/Discretize.cpp:844-1294), per-element
+ * pair weights w[nt][6] (2/(3V) x dihedral factor, src/Discretize.cpp:1355-1370). */
+int tsem_space_ref_matrices(const tsem_space* sp, double* mass, double* stiff6, double* pair_w);
+
+/* Boundary marks per vertex, edge, face: TSEM::build_flag_bm(mesh)
+ * (src/Discretize.cpp:107-118). */
+int tsem_set_marks(tsem_space* sp, const int32_t* mark_v, const int32_t* mark_e, const int32_t* mark_f);
+
+/* Points where boundary data are sampled (host): pv[nv][3]; pe[ne][q][3]
+ * along the first-visit edge direction; pf[nf][q*q][3] in the first-visit
+ * face order (src/Discretize.cpp:1797-1830). */
+int tsem_bc_points(const tsem_space* sp, double* pv, double* pe, double* pf);
+
+/* Load vector, device: b = sum_e G_e^T V_e B^T W f_e  (TSEM::calc_rhs with
+ * values at quadrature points, src/Discretize.cpp:1396-1418).  Call
+ * tsem_rhs_begin, then tsem_rhs_add for element ranges with f at the 3-D
+ * quadrature points (device, (e1-e0) x n_qp3), then tsem_rhs_end(b). */
+int tsem_qp_coords(tsem_space* sp, int e0, int e1, double* xyz_dev);
+int tsem_rhs_begin(tsem_space* sp);
+int tsem_rhs_add(tsem_space* sp, int e0, int e1, const double* fq_dev);
+int tsem_rhs_end(tsem_space* sp, double* b_dev);
+
+/* Neumann load on faces with mark == flag; g at face quadrature points
+ * (host, nf x q*q; rows of other faces ignored).  b device, updated.
+ * TSEM::impose_boundary_condition(rhs, mesh, g, flag) (src/Discretize.cpp:2010-2063). */
+int tsem_neumann(tsem_space* sp, const double* g_face_qp, int flag, double* b_dev);
+
+/* Dirichlet imposition on geometries with mark == flag: modal interpolation of
+ * the data (uv[nv], ue[ne][q], uf[nf][q*q] host, at tsem_bc_points), b
+ * updated and the operator switched to the eliminated form (rows/columns of
+ * the constrained DOFs replaced by the diagonal).  Optionally returns the
+ * imposed modal values (bval host, n_dof, NaN on free DOFs).
+ * TSEM::impose_boundary_condition(A, rhs, mesh, u, true, flag)
+ * (src/Discretize.cpp:1778-1893). */
+int tsem_dirichlet(tsem_space* sp, const double* uv, const double* ue, const double* uf, int flag, double* b_dev,
+                   double* bval);
+/* Zero Dirichlet elimination (operator only) for mark == flag:
+ * impose_zero_boundary_condition(A, flag) (src/Discretize.cpp:1465-1502). */
+int tsem_zero_bc(tsem_space* sp, int flag);
+
+/* Matrix-free operator, device: y = (stiff * S + mass * M) x with the
+ * Dirichlet elimination when constrained != 0.  prec 64: double vectors,
+ * 32: float vectors.  Replaces SparseMatrix<double>::vmult on the assembled
+ * TSEM::stiff_matrix / mass_matrix (src/TSEMSolver.cpp:1842). */
+int tsem_apply(tsem_space* sp, double stiff, double mass, int constrained, int prec, const void* x, void* y);
+
+/* Element-local coefficients, device: xl[nt][K] (K1 gather). */
+int tsem_gather(tsem_space* sp, const double* x_dev, double* xl_dev);
+/* Transposed gather, device: y = sum_e G_e^T yl_e (K1 scatter, fixed order). */
+int tsem_scatter(tsem_space* sp, const double* yl_dev, double* y_dev);
+
+/* Assembled diagonal (device copy out, n_dof). */
+int tsem_diagonal(tsem_space* sp, double* diag_dev);
+
+typedef struct {
+  double tol;          /* tolerance */
+  int criterion;       /* 0: reference, ||r||_1 < tol (absolute); 1: ||b-Ax||_2/||b||_2 <= tol */
+  int max_iter;        /* iteration limit (outer for mixed) */
+  int precond;         /* 1: Jacobi (reference default); 4: p-multigrid */
+  int mixed;           /* 0: FP64 PCG; 1: FP32 inner PCG inside FP64 iterative refinement */
+  double inner_tol;    /* mixed: relative reduction per inner solve */
+  int inner_max;       /* mixed: inner iteration limit */
+  int smooth_steps;    /* p-MG smoother degree */
+  int check_every;     /* host convergence check interval (iterations) */
+} tsem_pcg_opts;
+
+typedef struct {
+  int iterations;      /* total (inner for mixed) PCG iterations */
+  int outer;           /* refinement steps (mixed) */
+  double relres;       /* final true ||b-Ax||_2/||b||_2 (FP64) */
+  double l1;           /* final ||r||_1 of the recursive residual */
+  double seconds;      /* wall time of the solve (b and x resident) */
+} tsem_pcg_stats;
+
+/* PCG on the eliminated operator, device b/x (x: initial guess, solution).
+ * TSEMSolver<double>::solve_PCG(A, x, b, tol, max_step, flag, ite_step, var)
+ * (include/TSEMSolver.h:89-90, src/TSEMSolver.cpp:1775-1930). */
+int tsem_pcg(tsem_space* sp, const double* b_dev, double* x_dev, const tsem_pcg_opts* opts, tsem_pcg_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,131 @@
+"""Host side of the product library against the oracle (CPU only, no GPU).
+
+The product builds its index maps, DOF layout and reference matrices with its
+own code (hashing topology, exact integer sort keys, long-double reference
+element); a host-only space (no device context) is compared with the oracle
+here: integer maps bit-exact, reference data to round-off.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tsem_amd import api, lib, mesh
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "tsem.h")).read()
+    names = sorted(set(re.findall(r"\b(tsem_[a-z0-9_]+)\s*\(", hdr)))
+    L = lib.load()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(lib.SIGNATURES) <= set(names)
+
+
+CASES = [(1, 2, 0.0, 0), (2, 3, 0.1, 1), (2, 4, 0.1, 2), (3, 6, 0.15, 3), (2, 8, 0.1, 4), (2, 10, 0.05, 5)]
+
+
+@pytest.mark.parametrize("n,M,jit,seed", CASES)
+def test_host_space_matches_oracle(n, M, jit, seed):
+    xyz, tet = mesh.kuhn_cube(n, jitter=jit, seed=seed)
+    o = O.OracleSpace(M, xyz, tet)
+    p = api.Space(None, M, xyz, tet, qorder=M + 3)
+    assert (o.ndof, o.K, o.nv, o.ne, o.nf, o.nt) == (p.ndof, p.K, p.nv, p.ne, p.nf, p.nt)
+    for a, b in zip(o.topology(), p.topology()):
+        assert np.array_equal(a, b)
+    for a, b in zip(o.maps(), p.maps()):
+        assert np.array_equal(a, b)
+    assert np.array_equal(o.dof_start(), p.dof_start())
+    assert np.array_equal(o.sum_dof(), p.sum_dof())
+    Mo, So = o.ref_matrices()
+    Mp, Sp, wp = p.ref_matrices()
+    assert np.abs(Mo - Mp).max() <= 1e-14 * np.abs(Mo).max()
+    assert np.abs(So - Sp).max() <= 1e-13 * np.abs(So).max()
+    assert np.array_equal(o.stiff_weights(), wp)
+    o.assemble(1.0, 0.0)
+    rs, col, v = o.csr()
+    assert np.abs(v[rs[:-1]] - p.diagonal_host()).max() <= 1e-14 * np.abs(v[rs[:-1]]).max()
+
+
+def test_custom_femdof_order():
+    """A caller-supplied P2 DOF order (as AFEPack would give) drives the
+    layout identically in oracle and product."""
+    xyz, tet = mesh.kuhn_cube(2, jitter=0.1, seed=3)
+    o0 = O.OracleSpace(4, xyz, tet)
+    nw = o0.nv + o0.ne
+    w = np.random.default_rng(7).permutation(nw).astype(np.int32)
+    o = O.OracleSpace(4, xyz, tet, femdof_weight=w)
+    p = api.Space(None, 4, xyz, tet, femdof_weight=w, qorder=7)
+    assert np.array_equal(o.dof_start(), p.dof_start())
+    for a, b in zip(o.maps(), p.maps()):
+        assert np.array_equal(a, b)
+    assert not np.array_equal(o.dof_start(), o0.dof_start())
+
+
+def test_single_tetrahedron_and_permuted_vertices():
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    for perm in ([0, 1, 2, 3], [2, 0, 3, 1], [3, 2, 1, 0]):
+        tet = np.array([perm], np.int32)
+        o = O.OracleSpace(5, xyz, tet)
+        p = api.Space(None, 5, xyz, tet, qorder=8)
+        assert p.ndof == o.ndof == 56
+        for a, b in zip(o.maps(), p.maps()):
+            assert np.array_equal(a, b)
+
+
+def test_invalid_inputs_raise():
+    xyz, tet = mesh.kuhn_cube(1)
+    with pytest.raises(lib.TsemError) as e:
+        api.Space(None, 0, xyz, tet)
+    assert e.value.code == lib.TSEM_EINVAL
+    with pytest.raises(lib.TsemError):
+        api.Space(None, 13, xyz, tet)
+    bad = tet.copy()
+    bad[0, 0] = len(xyz) + 5
+    with pytest.raises(lib.TsemError) as e:
+        api.Space(None, 3, xyz, bad)
+    assert e.value.code == lib.TSEM_EINVAL
+    flat = xyz.copy()
+    flat[:, 2] = 0.0  # every tet degenerate
+    with pytest.raises(lib.TsemError) as e:
+        api.Space(None, 3, flat, tet)
+    assert e.value.code == lib.TSEM_EINVAL
+
+
+def test_host_only_space_refuses_device_work():
+    xyz, tet = mesh.kuhn_cube(1)
+    p = api.Space(None, 3, xyz, tet, qorder=5)
+    with pytest.raises(lib.TsemError) as e:
+        import ctypes as C
+        lib.check(lib.load().tsem_rhs_begin(p.h))
+    assert e.value.code == lib.TSEM_ESTATE
+
+
+def test_afepack_mesh_roundtrip(tmp_path):
+    xyz, tet = mesh.kuhn_cube(2, jitter=0.1, seed=1)
+    p = api.Space(None, 3, xyz, tet, qorder=5)
+    ev, fv, fe, tf = p.topology()
+    marks = mesh.geometry_marks(mesh.cube_plane_marks(xyz), ev, fv)
+    path = tmp_path / "cube.mesh"
+    mesh.write_afepack_mesh(str(path), xyz, ev, fv, fe, tet, tf, marks)
+    xyz2, tet2, marks2, topo2 = mesh.read_afepack_mesh(str(path))
+    assert np.array_equal(xyz, xyz2) and np.array_equal(tet, tet2)
+    for a, b in zip(marks, marks2):
+        assert np.array_equal(a, b)
+    for a, b in zip((ev, fv, fe, tf), topo2):
+        assert np.array_equal(a, b)
+
+
+def test_mesh_generators():
+    xyz, tet = mesh.kuhn_cube(4, jitter=0.1, seed=2026)
+    assert len(tet) == 6 * 64 and len(xyz) == 125
+    p = xyz[tet]
+    det = np.einsum("ij,ij->i", p[:, 1] - p[:, 0], np.cross(p[:, 2] - p[:, 0], p[:, 3] - p[:, 0]))
+    assert det.min() > 0
+    assert abs(det.sum() / 6 - 1.0) < 1e-12  # tets tile the cube
+    g, t = mesh.graded_box(6)
+    assert abs(g.min() + 10) < 1e-12 and abs(g.max() - 10) < 1e-12

@@ -1,0 +1,217 @@
+"""Python host side of the TSEM library: spaces, operators and solvers.
+
+Device vectors are torch tensors on ``cuda:<device>`` (HIP on ROCm); the work
+itself runs in libtsem_amd.so through the C ABI.  ``Space(None, ...)`` builds
+a host-only space (index maps, reference matrices, diagonal) without a GPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import lib as L
+from .mesh import cube_plane_marks, geometry_marks
+
+
+class Context:
+    """Device + stream (tsem_ctx_create)."""
+
+    def __init__(self, device=0):
+        self.device = device
+        h = C.c_void_p()
+        L.check(L.load().tsem_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.load().tsem_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+class Space:
+    """TSEM space of order M on a tetrahedral mesh (TSEM<double> + init)."""
+
+    def __init__(self, ctx, M, xyz, tet, femdof_weight=None, qorder=30):
+        lib = L.load()
+        self.ctx = ctx
+        self.M = M
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.tet = np.ascontiguousarray(tet, dtype=np.int32)
+        fw = None if femdof_weight is None else np.ascontiguousarray(femdof_weight, dtype=np.int32)
+        h = C.c_void_p()
+        L.check(lib.tsem_space_create(ctx.h if ctx is not None else None, M, len(self.xyz), L.dptr(self.xyz),
+                                      len(self.tet), L.iptr(self.tet), L.iptr(fw) if fw is not None else None,
+                                      qorder, C.byref(h)))
+        self.h = h
+        info = np.zeros(12, np.int64)
+        L.check(lib.tsem_space_info(h, L.lptr(info)))
+        self.ndof, self.K, self.nv, self.ne, self.nf, self.nt = (int(v) for v in info[:6])
+        self.ndg = [int(v) for v in info[6:10]]
+        self.nmap = int(info[10])
+        self.nq3 = int(info[11])
+        self.qorder = qorder
+        self.marks = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.load().tsem_space_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def device(self):
+        import torch
+        return torch.device("cuda", self.ctx.device)
+
+    # ---------------------------------------------------------------- host --
+    def topology(self):
+        ev = np.zeros((self.ne, 2), np.int32)
+        fv = np.zeros((self.nf, 3), np.int32)
+        fe = np.zeros((self.nf, 3), np.int32)
+        tf = np.zeros((self.nt, 4), np.int32)
+        L.check(L.load().tsem_space_topology(self.h, L.iptr(ev), L.iptr(fv), L.iptr(fe), L.iptr(tf)))
+        return ev, fv, fe, tf
+
+    def maps(self):
+        off = np.zeros(self.nt * self.K + 1, np.int32)
+        ind = np.zeros(self.nmap, np.int32)
+        val = np.zeros(self.nmap)
+        L.check(L.load().tsem_space_maps(self.h, L.iptr(off), L.iptr(ind), L.dptr(val)))
+        return off, ind, val
+
+    def dof_start(self):
+        out = np.zeros(self.nv + self.ne + self.nf + self.nt, np.int64)
+        L.check(L.load().tsem_space_dof_start(self.h, L.lptr(out)))
+        return out
+
+    def sum_dof(self):
+        out = np.zeros(self.ndof, np.int32)
+        L.check(L.load().tsem_space_sum_dof(self.h, L.iptr(out)))
+        return out
+
+    def ref_matrices(self):
+        K = self.K
+        Mr = np.zeros((K, K)); S = np.zeros((6, K, K)); w = np.zeros((self.nt, 6))
+        L.check(L.load().tsem_space_ref_matrices(self.h, L.dptr(Mr), L.dptr(S), L.dptr(w)))
+        return Mr, S, w
+
+    def diagonal_host(self):
+        d = np.zeros(self.ndof)
+        L.check(L.load().tsem_space_diagonal_host(self.h, L.dptr(d)))
+        return d
+
+    def set_marks(self, mv, me, mf):
+        """build_flag_bm(mesh): boundary mark per vertex, edge, face."""
+        self.marks = [np.ascontiguousarray(m, dtype=np.int32) for m in (mv, me, mf)]
+        L.check(L.load().tsem_set_marks(self.h, *(L.iptr(m) for m in self.marks)))
+
+    def set_cube_marks(self, dirichlet_planes=0b000001, neumann_planes=0b111110):
+        ev, fv, _, _ = self.topology()
+        mv, me, mf = geometry_marks(cube_plane_marks(self.xyz), ev, fv, dirichlet_planes, neumann_planes)
+        self.set_marks(mv, me, mf)
+        return mv, me, mf
+
+    def bc_points(self):
+        q = self.qorder
+        pv = np.zeros((self.nv, 3)); pe = np.zeros((self.ne, q, 3)); pf = np.zeros((self.nf, q * q, 3))
+        L.check(L.load().tsem_bc_points(self.h, L.dptr(pv), L.dptr(pe), L.dptr(pf)))
+        return pv, pe, pf
+
+    # -------------------------------------------------------------- device --
+    def zeros(self, dtype=None):
+        import torch
+        return torch.zeros(self.ndof, dtype=dtype or torch.float64, device=self.device)
+
+    def rhs(self, f, chunk=None):
+        """calc_rhs: load vector of a source term f(points) evaluated on device
+        at the 3-D quadrature points, chunked over elements."""
+        import torch
+        lib = L.load()
+        b = self.zeros()
+        L.check(lib.tsem_rhs_begin(self.h))
+        if chunk is None:
+            chunk = max(1, min(self.nt, int(2e8 // max(1, self.nq3 * 3))))
+        for e0 in range(0, self.nt, chunk):
+            e1 = min(self.nt, e0 + chunk)
+            pts = torch.empty((e1 - e0, self.nq3, 3), dtype=torch.float64, device=self.device)
+            L.check(lib.tsem_qp_coords(self.h, e0, e1, _ptr(pts)))
+            fq = f(pts).contiguous()
+            L.check(lib.tsem_rhs_add(self.h, e0, e1, _ptr(fq)))
+            del pts, fq
+        L.check(lib.tsem_rhs_end(self.h, _ptr(b)))
+        return b
+
+    def neumann(self, g_face_qp, flag, b):
+        g = np.ascontiguousarray(g_face_qp, dtype=np.float64)
+        L.check(L.load().tsem_neumann(self.h, L.dptr(g), flag, _ptr(b)))
+
+    def dirichlet(self, uv, ue, uf, flag, b):
+        uv, ue, uf = (np.ascontiguousarray(a, dtype=np.float64) for a in (uv, ue, uf))
+        bval = np.zeros(self.ndof)
+        L.check(L.load().tsem_dirichlet(self.h, L.dptr(uv), L.dptr(ue), L.dptr(uf), flag, _ptr(b), L.dptr(bval)))
+        return bval
+
+    def zero_bc(self, flag):
+        L.check(L.load().tsem_zero_bc(self.h, flag))
+
+    def apply(self, x, stiff=1.0, mass=0.0, constrained=False, out=None):
+        import torch
+        prec = 64 if x.dtype == torch.float64 else 32
+        y = torch.empty_like(x) if out is None else out
+        L.check(L.load().tsem_apply(self.h, stiff, mass, 1 if constrained else 0, prec, _ptr(x), _ptr(y)))
+        return y
+
+    def gather(self, x):
+        import torch
+        xl = torch.empty((self.nt, self.K), dtype=torch.float64, device=self.device)
+        L.check(L.load().tsem_gather(self.h, _ptr(x), _ptr(xl)))
+        return xl
+
+    def scatter(self, yl):
+        y = self.zeros()
+        L.check(L.load().tsem_scatter(self.h, _ptr(yl.contiguous()), _ptr(y)))
+        return y
+
+    def diagonal(self):
+        d = self.zeros()
+        L.check(L.load().tsem_diagonal(self.h, _ptr(d)))
+        return d
+
+    def pcg(self, b, x=None, tol=1e-10, criterion=1, max_iter=100000, precond=1, mixed=False, inner_tol=1e-4,
+            inner_max=100000, smooth_steps=2, check_every=1, raise_on_noconv=True):
+        """solve_PCG on the eliminated operator; returns (x, stats dict)."""
+        x = self.zeros() if x is None else x
+        o = L.PcgOpts(tol, criterion, max_iter, precond, 1 if mixed else 0, inner_tol, inner_max, smooth_steps,
+                      check_every)
+        s = L.PcgStats()
+        rc = L.load().tsem_pcg(self.h, _ptr(b), _ptr(x), C.byref(o), C.byref(s))
+        if rc != L.TSEM_OK and (rc != L.TSEM_ENOCONV or raise_on_noconv):
+            L.check(rc)
+        return x, dict(iterations=s.iterations, outer=s.outer, relres=s.relres, l1=s.l1, seconds=s.seconds,
+                       converged=(rc == L.TSEM_OK))
+
+
+def poisson_setup(sp, problem, dirichlet_planes=0b000001, neumann_planes=0b111110):
+    """Mixed-BC Poisson system on the unit cube, the sequence of
+    examples/poisson_mixBC/poisson.cpp:74-89: load vector, Neumann load on
+    mark 2, Dirichlet elimination on mark 1.  Returns b (device)."""
+    import torch
+    from .problems import cube_face_normal
+    sp.set_cube_marks(dirichlet_planes, neumann_planes)
+    b = sp.rhs(lambda p: problem.f(p))
+    pv, pe, pf = sp.bc_points()
+    ev, fv, _, _ = sp.topology()
+    normals = cube_face_normal(sp.xyz[fv])
+    gq = problem.dudn(pf, [normals[:, None, d] for d in range(3)])
+    sp.neumann(gq, 2, b)
+    sp.bval = sp.dirichlet(problem.u(pv), problem.u(pe), problem.u(pf), 1, b)
+    torch.cuda.synchronize()
+    return b

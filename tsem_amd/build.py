@@ -1,0 +1,49 @@
+"""Build libtsem_amd.so (host C++ + HIP kernels for gfx950) in-tree.
+
+    python -m tsem_amd.build          # or tsem_amd.build.build()
+
+Every source is compiled with hipcc for --offload-arch=gfx950; packed FP32
+instructions are disabled (-packed-fp32-ops) for all device code.
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libtsem_amd.so")
+SOURCES = ["refel.cpp", "space.cpp", "capi.cpp", "kernels.hip", "pmg.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wall", "-Wno-unused-result", "-Wno-unused-variable"]
+
+
+def _compile(src, obj):
+    lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
+    cmd = [HIPCC] + FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+    subprocess.check_call(cmd)
+
+
+def build(jobs=None, verbose=True):
+    """Compile all sources (in parallel) and link the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+    tmp = tempfile.mkdtemp(prefix="tsem_build_")
+    objs = [os.path.join(tmp, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    with ThreadPoolExecutor(max_workers=jobs or min(8, len(SOURCES))) as ex:
+        list(ex.map(lambda a: _compile(*a), zip(SOURCES, objs)))
+    out_tmp = LIB + ".tmp"
+    subprocess.check_call([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", out_tmp] + objs + ["-ldl"])
+    os.replace(out_tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    os.rmdir(tmp)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,848 @@
+// C ABI of the TSEM library (include/tsem.h): contexts, spaces, device
+// buffers, boundary conditions and the PCG / mixed-precision drivers.
+#include "../../include/tsem.h"
+
+#include <hip/hip_runtime.h>
+
+#include <dlfcn.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "pmg.hpp"
+#include "space.hpp"
+
+using namespace tsem;
+
+static thread_local std::string g_err;
+
+const char* tsem_last_error(void) { return g_err.c_str(); }
+
+#define HIPCHK(x)                                                                      \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP: ") + hipGetErrorString(e_) + " at " #x); \
+  } while (0)
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  void alloc(size_t bytes) {
+    if (p && n >= bytes) return;
+    if (p) { (void)hipFree(p); p = nullptr; }
+    if (bytes == 0) bytes = 8;
+    if (hipMalloc(&p, bytes) != hipSuccess) { p = nullptr; throw std::bad_alloc(); }
+    n = bytes;
+  }
+  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+template <typename T>
+void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+  b.alloc(v.size() * sizeof(T));
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+// standard Jacobi P_n^{(a,b)}(x), a,b > -1
+double jacobi(double a, double b, int n, double x) {
+  if (n == 0) return 1.0;
+  double p0 = 1.0, p1 = (a - b) / 2 + (a + b + 2) / 2 * x;
+  for (int k = 2; k <= n; ++k) {
+    double c = 2 * k + a + b;
+    double A1 = 2 * k * (k + a + b) * (c - 2);
+    double A2 = (c - 1) * (a * a - b * b);
+    double A3 = (c - 1) * c * (c - 2);
+    double A4 = 2 * (k + a - 1) * (k + b - 1) * c;
+    double p2 = ((A2 + A3 * x) * p1 - A4 * p0) / A1;
+    p0 = p1; p1 = p2;
+  }
+  return p1;
+}
+
+}  // namespace
+
+struct tsem_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+struct tsem_space {
+  tsem_ctx* ctx = nullptr;
+  Space h;
+  int qorder = 0, nq1 = 0, nq2 = 0, nq3 = 0;
+  std::vector<double> t1, w1, p2, w2, p3, w3;  // BC/RHS quadrature
+  std::vector<double> diag;                     // host copy
+  std::vector<uint8_t> cmask;
+  DevSpace d;
+  // device storage
+  DevBuf desc, slot, ft_ptr, ft_key, ft_w, ghat, ghat32, sixv, sixv32, dT, dN, dT32, dN32, mT, mN, mT32, mN32;
+  DevBuf sc_ptr, sc_code, sc_w, diag_d, invdiag, invdiag32, cmask_d, xe, pts3, wphi;
+  DevBuf yl64, yl32, bl, partial, scal;
+  // solver vectors
+  DevBuf v64[6], v32[6];
+  std::unique_ptr<Pmg> pmg;
+  bool rhs_open = false;
+  hipStream_t st() const {
+    if (!ctx) throw std::logic_error("space was created without a device context");
+    return ctx->stream;
+  }
+};
+
+#define API_BEGIN try {
+#define API_END                                         \
+  }                                                     \
+  catch (const std::bad_alloc&) {                       \
+    g_err = "allocation failed";                        \
+    return TSEM_ENOMEM;                                 \
+  }                                                     \
+  catch (const std::invalid_argument& e) {              \
+    g_err = e.what();                                   \
+    return TSEM_EINVAL;                                 \
+  }                                                     \
+  catch (const std::logic_error& e) {                   \
+    g_err = e.what();                                   \
+    return TSEM_ESTATE;                                 \
+  }                                                     \
+  catch (const std::exception& e) {                     \
+    g_err = e.what();                                   \
+    return TSEM_EDEVICE;                                \
+  }                                                     \
+  return TSEM_OK;
+
+// the face table ships next to the library: <dir of libtsem_amd.so>/data/face_table.txt
+static std::string face_table_path() {
+  const char* env = getenv("TSEM_FACE_TABLE");
+  if (env) return env;
+  Dl_info info;
+  if (dladdr((void*)&face_table_path, &info) && info.dli_fname) {
+    std::string f(info.dli_fname);
+    size_t k = f.find_last_of('/');
+    std::string dir = (k == std::string::npos) ? std::string(".") : f.substr(0, k);
+    return dir + "/data/face_table.txt";
+  }
+  return "data/face_table.txt";
+}
+
+extern "C" {
+
+int tsem_ctx_create(int device, tsem_ctx** out) {
+  API_BEGIN
+  if (!out) throw std::invalid_argument("null output");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) throw std::invalid_argument("device index out of range");
+  HIPCHK(hipSetDevice(device));
+  auto* c = new tsem_ctx();
+  c->device = device;
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = c;
+  API_END
+}
+
+int tsem_ctx_destroy(tsem_ctx* ctx) {
+  API_BEGIN
+  if (!ctx) return TSEM_OK;
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  API_END
+}
+
+int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, const int32_t* tet,
+                      const int32_t* femdof_weight, int qorder, tsem_space** out) {
+  API_BEGIN
+  if (!out || !xyz || !tet) throw std::invalid_argument("null argument");
+  if (M < 1 || M > 12) throw std::invalid_argument("polynomial order must be in 1..12");
+  if (nv < 4 || nt < 1) throw std::invalid_argument("empty mesh");
+  if (qorder < 1 || qorder > 64) throw std::invalid_argument("qorder must be in 1..64");
+  if (ctx) HIPCHK(hipSetDevice(ctx->device));
+  auto sp = std::make_unique<tsem_space>();
+  sp->ctx = ctx;
+  Space& S = sp->h;
+  try {
+    S.build(M, nv, xyz, nt, tet, femdof_weight, face_table_path());
+  } catch (const std::runtime_error& e) {
+    throw std::invalid_argument(e.what());
+  }
+  for (int t = 0; t < nt; ++t)
+    if (!(S.vol[t] > 0)) throw std::invalid_argument("degenerate tetrahedron");
+  // quadrature
+  sp->qorder = qorder;
+  std::vector<long double> t, w;
+  gauss_legendre01(qorder, t, w);
+  int q = qorder;
+  sp->nq1 = q; sp->nq2 = q * q; sp->nq3 = q * q * q;
+  sp->t1.resize(q); sp->w1.resize(q);
+  for (int i = 0; i < q; ++i) { sp->t1[i] = (double)t[i]; sp->w1[i] = (double)w[i]; }
+  sp->p2.resize(3 * (size_t)q * q); sp->w2.resize((size_t)q * q);
+  for (int i = 0; i < q; ++i)
+    for (int j = 0; j < q; ++j) {
+      size_t k = (size_t)i * q + j;
+      long double a = t[i] * (1 - t[j]), b = t[j];
+      sp->p2[3 * k] = (double)a; sp->p2[3 * k + 1] = (double)b; sp->p2[3 * k + 2] = (double)(1 - a - b);
+      sp->w2[k] = (double)(2 * (1 - t[j]) * w[i] * w[j]);
+    }
+  sp->p3.resize(3 * (size_t)sp->nq3); sp->w3.resize(sp->nq3);
+  for (int i = 0; i < q; ++i)
+    for (int j = 0; j < q; ++j)
+      for (int l = 0; l < q; ++l) {
+        size_t k = ((size_t)i * q + j) * q + l;
+        sp->p3[3 * k] = (double)(t[i] * (1 - t[j]) * (1 - t[l]));
+        sp->p3[3 * k + 1] = (double)(t[j] * (1 - t[l]));
+        sp->p3[3 * k + 2] = (double)t[l];
+        sp->w3[k] = (double)(6 * (1 - t[j]) * (1 - t[l]) * (1 - t[l]) * w[i] * w[j] * w[l]);
+      }
+  S.diagonal(1.0, 0.0, sp->diag);
+  sp->cmask.assign(S.ndof, 0);
+  if (!ctx) {  // host-only space: setup and map export without a device
+    *out = sp.release();
+    return TSEM_OK;
+  }
+  hipStream_t st = ctx->stream;
+  // device data
+  DevSpace& d = sp->d;
+  d.nt = nt; d.K = S.K; d.Kd = S.ref.Kd; d.M = M; d.nface = S.table.nface; d.ndof = (int)S.ndof;
+  upload(sp->desc, S.desc, st);
+  std::vector<int32_t> slot(S.K);
+  for (int i = 0; i < S.K; ++i) slot[i] = S.slot_kind[i] | (S.slot_geo[i] << 2) | (S.slot_sub[i] << 5);
+  upload(sp->slot, slot, st);
+  std::vector<int32_t> fptr(1, 0), fkey;
+  std::vector<double> fw;
+  for (auto& row : S.table.terms) {
+    for (auto& tm : row) { fkey.push_back(tm.ie | (tm.idx << 2)); fw.push_back(tm.w); }
+    fptr.push_back((int32_t)fkey.size());
+  }
+  upload(sp->ft_ptr, fptr, st); upload(sp->ft_key, fkey, st); upload(sp->ft_w, fw, st);
+  upload(sp->ghat, S.ghat, st);
+  upload(sp->ghat32, std::vector<float>(S.ghat.begin(), S.ghat.end()), st);
+  upload(sp->sixv, S.sixv, st);
+  upload(sp->sixv32, std::vector<float>(S.sixv.begin(), S.sixv.end()), st);
+  int K = S.K, Kd = S.ref.Kd, R = 3 * Kd;
+  std::vector<double> dT((size_t)K * R), dN((size_t)R * K);
+  for (int a = 0; a < 3; ++a)
+    for (int m = 0; m < Kd; ++m)
+      for (int i = 0; i < K; ++i) {
+        double v = S.ref.dfac[a][(size_t)m * K + i];
+        dT[(size_t)i * R + a * Kd + m] = v;
+        dN[(size_t)(a * Kd + m) * K + i] = v;
+      }
+  std::vector<double> mT((size_t)K * K), mN(S.ref.mfac);
+  for (int m = 0; m < K; ++m)
+    for (int i = 0; i < K; ++i) mT[(size_t)i * K + m] = S.ref.mfac[(size_t)m * K + i];
+  upload(sp->dT, dT, st); upload(sp->dN, dN, st); upload(sp->mT, mT, st); upload(sp->mN, mN, st);
+  upload(sp->dT32, std::vector<float>(dT.begin(), dT.end()), st);
+  upload(sp->dN32, std::vector<float>(dN.begin(), dN.end()), st);
+  upload(sp->mT32, std::vector<float>(mT.begin(), mT.end()), st);
+  upload(sp->mN32, std::vector<float>(mN.begin(), mN.end()), st);
+  std::vector<int32_t> scp(S.sc_ptr.size());
+  if (S.sc_ptr.back() >= (1LL << 31)) throw std::invalid_argument("scatter map too large");
+  for (size_t i = 0; i < scp.size(); ++i) scp[i] = (int32_t)S.sc_ptr[i];
+  upload(sp->sc_ptr, scp, st); upload(sp->sc_code, S.sc_code, st); upload(sp->sc_w, S.sc_w, st);
+  std::vector<double> inv(sp->diag.size());
+  for (size_t i = 0; i < inv.size(); ++i) inv[i] = 1.0 / sp->diag[i];
+  upload(sp->diag_d, sp->diag, st); upload(sp->invdiag, inv, st);
+  upload(sp->invdiag32, std::vector<float>(inv.begin(), inv.end()), st);
+  upload(sp->cmask_d, sp->cmask, st);
+  std::vector<double> xe((size_t)nt * 12);
+  for (int e = 0; e < nt; ++e)
+    for (int k = 0; k < 4; ++k)
+      for (int c = 0; c < 3; ++c) xe[(size_t)e * 12 + 3 * k + c] = xyz[3 * (size_t)tet[4 * e + k] + c];
+  upload(sp->xe, xe, st);
+  upload(sp->pts3, sp->p3, st);
+  sp->yl64.alloc((size_t)nt * K * sizeof(double));
+  sp->yl32.alloc((size_t)nt * K * sizeof(float));
+  sp->partial.alloc((size_t)4 * (reduce_blocks((int)S.ndof) + 16) * sizeof(double));
+  sp->scal.alloc(sizeof(CgScalars));
+  d.desc = sp->desc.as<ElemDesc>(); d.slot = sp->slot.as<int32_t>();
+  d.ft_ptr = sp->ft_ptr.as<int32_t>(); d.ft_key = sp->ft_key.as<int32_t>(); d.ft_w = sp->ft_w.as<double>();
+  d.ghat = sp->ghat.as<double>(); d.ghat32 = sp->ghat32.as<float>();
+  d.sixv = sp->sixv.as<double>(); d.sixv32 = sp->sixv32.as<float>();
+  d.dT = sp->dT.as<double>(); d.dN = sp->dN.as<double>(); d.dT32 = sp->dT32.as<float>(); d.dN32 = sp->dN32.as<float>();
+  d.mT = sp->mT.as<double>(); d.mN = sp->mN.as<double>(); d.mT32 = sp->mT32.as<float>(); d.mN32 = sp->mN32.as<float>();
+  d.sc_ptr = sp->sc_ptr.as<int32_t>(); d.sc_code = sp->sc_code.as<int32_t>(); d.sc_w = sp->sc_w.as<double>();
+  d.diag = sp->diag_d.as<double>(); d.invdiag = sp->invdiag.as<double>(); d.invdiag32 = sp->invdiag32.as<float>();
+  d.cmask = sp->cmask_d.as<uint8_t>();
+  HIPCHK(hipStreamSynchronize(st));
+  *out = sp.release();
+  API_END
+}
+
+int tsem_space_destroy(tsem_space* sp) {
+  API_BEGIN
+  if (sp) {
+    if (sp->ctx) (void)hipStreamSynchronize(sp->ctx->stream);
+    delete sp;
+  }
+  API_END
+}
+
+int tsem_space_diagonal_host(const tsem_space* sp, double* diag) {
+  API_BEGIN
+  if (!sp || !diag) throw std::invalid_argument("null argument");
+  std::memcpy(diag, sp->diag.data(), sp->diag.size() * 8);
+  API_END
+}
+
+int tsem_space_info(const tsem_space* sp, int64_t* info) {
+  API_BEGIN
+  if (!sp || !info) throw std::invalid_argument("null argument");
+  const Space& S = sp->h;
+  info[0] = S.ndof; info[1] = S.K; info[2] = S.topo.nv; info[3] = S.topo.ne; info[4] = S.topo.nf; info[5] = S.topo.nt;
+  for (int k = 0; k < 4; ++k) info[6 + k] = S.ndg[k];
+  std::vector<std::pair<int64_t, double>> tm;
+  int64_t n = 0;
+  for (int e = 0; e < S.topo.nt; ++e)
+    for (int i = 0; i < S.K; ++i) { tm.clear(); S.gather_terms(e, i, tm); n += (int64_t)tm.size(); }
+  info[10] = n;
+  info[11] = sp->nq3;
+  API_END
+}
+
+int tsem_space_topology(const tsem_space* sp, int32_t* ev, int32_t* fv, int32_t* fe, int32_t* tf) {
+  API_BEGIN
+  if (!sp) throw std::invalid_argument("null space");
+  const Topology& T = sp->h.topo;
+  if (ev) std::memcpy(ev, T.edge_vtx.data(), T.edge_vtx.size() * 4);
+  if (fv) std::memcpy(fv, T.face_vtx.data(), T.face_vtx.size() * 4);
+  if (fe) std::memcpy(fe, T.face_edge.data(), T.face_edge.size() * 4);
+  if (tf) std::memcpy(tf, T.tet_face.data(), T.tet_face.size() * 4);
+  API_END
+}
+
+int tsem_space_maps(const tsem_space* sp, int32_t* off, int32_t* ind, double* val) {
+  API_BEGIN
+  if (!sp || !off || !ind || !val) throw std::invalid_argument("null argument");
+  const Space& S = sp->h;
+  std::vector<std::pair<int64_t, double>> tm;
+  int64_t n = 0;
+  off[0] = 0;
+  for (int e = 0; e < S.topo.nt; ++e)
+    for (int i = 0; i < S.K; ++i) {
+      tm.clear();
+      S.gather_terms(e, i, tm);
+      for (auto& pr : tm) { ind[n] = (int32_t)pr.first; val[n] = pr.second; ++n; }
+      off[(size_t)e * S.K + i + 1] = (int32_t)n;
+    }
+  API_END
+}
+
+int tsem_space_dof_start(const tsem_space* sp, int64_t* start) {
+  API_BEGIN
+  if (!sp || !start) throw std::invalid_argument("null argument");
+  std::memcpy(start, sp->h.dof_start.data(), sp->h.dof_start.size() * 8);
+  API_END
+}
+
+int tsem_space_sum_dof(const tsem_space* sp, int32_t* sd) {
+  API_BEGIN
+  if (!sp || !sd) throw std::invalid_argument("null argument");
+  std::memcpy(sd, sp->h.sum_dof.data(), sp->h.sum_dof.size() * 4);
+  API_END
+}
+
+int tsem_space_ref_matrices(const tsem_space* sp, double* mass, double* stiff6, double* pw) {
+  API_BEGIN
+  if (!sp) throw std::invalid_argument("null space");
+  const Space& S = sp->h;
+  size_t KK = (size_t)S.K * S.K;
+  if (mass) std::memcpy(mass, S.ref.mass.data(), KK * 8);
+  if (stiff6) for (int k = 0; k < 6; ++k) std::memcpy(stiff6 + k * KK, S.ref.stiff[k].data(), KK * 8);
+  if (pw) std::memcpy(pw, S.pairw.data(), S.pairw.size() * 8);
+  API_END
+}
+
+int tsem_set_marks(tsem_space* sp, const int32_t* mv, const int32_t* me, const int32_t* mf) {
+  API_BEGIN
+  if (!sp || !mv || !me || !mf) throw std::invalid_argument("null argument");
+  Space& S = sp->h;
+  S.mark[0].assign(mv, mv + S.topo.nv);
+  S.mark[1].assign(me, me + S.topo.ne);
+  S.mark[2].assign(mf, mf + S.topo.nf);
+  API_END
+}
+
+int tsem_bc_points(const tsem_space* sp, double* pv, double* pe, double* pf) {
+  API_BEGIN
+  if (!sp) throw std::invalid_argument("null space");
+  const Space& S = sp->h;
+  int q = sp->nq1, q2 = sp->nq2;
+  if (pv) std::memcpy(pv, S.xyz.data(), S.xyz.size() * 8);
+  if (pe)
+    for (int e = 0; e < S.topo.ne; ++e) {
+      const double* a = &S.xyz[3 * (size_t)S.edge_node[e][0]];
+      const double* b = &S.xyz[3 * (size_t)S.edge_node[e][1]];
+      for (int k = 0; k < q; ++k)
+        for (int c = 0; c < 3; ++c) pe[((size_t)e * q + k) * 3 + c] = a[c] * (1 - sp->t1[k]) + b[c] * sp->t1[k];
+    }
+  if (pf)
+    for (int f = 0; f < S.topo.nf; ++f) {
+      const double* n0 = &S.xyz[3 * (size_t)S.face_node[f][0]];
+      const double* n1 = &S.xyz[3 * (size_t)S.face_node[f][1]];
+      const double* n2 = &S.xyz[3 * (size_t)S.face_node[f][2]];
+      for (int k = 0; k < q2; ++k) {
+        const double* bq = &sp->p2[3 * (size_t)k];
+        for (int c = 0; c < 3; ++c) pf[((size_t)f * q2 + k) * 3 + c] = n0[c] * bq[2] + n1[c] * bq[0] + n2[c] * bq[1];
+      }
+    }
+  API_END
+}
+
+int tsem_qp_coords(tsem_space* sp, int e0, int e1, double* out) {
+  API_BEGIN
+  if (!sp || !out || e0 < 0 || e1 > sp->h.topo.nt || e0 > e1) throw std::invalid_argument("bad element range");
+  launch_qp_coords(sp->d, sp->xe.as<double>(), e0, e1, sp->nq3, sp->pts3.as<double>(), out, sp->st());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
+int tsem_rhs_begin(tsem_space* sp) {
+  API_BEGIN
+  if (!sp) throw std::invalid_argument("null space");
+  const Space& S = sp->h;
+  if (!sp->wphi.p) {
+    std::vector<double> wphi((size_t)sp->nq3 * S.K);
+    std::vector<long double> val(S.K);
+    for (int q = 0; q < sp->nq3; ++q) {
+      S.ref.eval(sp->p3[3 * q], sp->p3[3 * q + 1], sp->p3[3 * q + 2], val.data(), nullptr);
+      for (int i = 0; i < S.K; ++i) wphi[(size_t)q * S.K + i] = sp->w3[q] * (double)val[i];
+    }
+    upload(sp->wphi, wphi, sp->st());
+  }
+  sp->bl.alloc((size_t)S.topo.nt * S.K * sizeof(double));
+  HIPCHK(hipMemsetAsync(sp->bl.p, 0, (size_t)S.topo.nt * S.K * sizeof(double), sp->st()));
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  sp->rhs_open = true;
+  API_END
+}
+
+int tsem_rhs_add(tsem_space* sp, int e0, int e1, const double* fq) {
+  API_BEGIN
+  if (!sp || !fq) throw std::invalid_argument("null argument");
+  if (!sp->rhs_open) { g_err = "tsem_rhs_begin not called"; return TSEM_ESTATE; }
+  if (e0 < 0 || e1 > sp->h.topo.nt || e0 > e1) throw std::invalid_argument("bad element range");
+  launch_rhs_local(sp->d, e0, e1, sp->nq3, sp->wphi.as<double>(), fq, sp->bl.as<double>(), sp->st());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
+int tsem_rhs_end(tsem_space* sp, double* b) {
+  API_BEGIN
+  if (!sp || !b) throw std::invalid_argument("null argument");
+  if (!sp->rhs_open) { g_err = "tsem_rhs_begin not called"; return TSEM_ESTATE; }
+  launch_scatter<double>(sp->d, sp->bl.as<double>(), b, sp->st());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  sp->rhs_open = false;
+  API_END
+}
+
+// ---------------------------------------------------------------- BCs ------
+int tsem_neumann(tsem_space* sp, const double* gq, int flag, double* b_dev) {
+  API_BEGIN
+  if (!sp || !gq || !b_dev) throw std::invalid_argument("null argument");
+  Space& S = sp->h;
+  if (S.mark[2].empty()) { g_err = "boundary marks not set"; return TSEM_ESTATE; }
+  int M = S.M, q2 = sp->nq2, nde = S.ndg[1], ndf = S.ndg[2];
+  // face-local bases at the 2-D points: face modes (TSEM basis_value[1]),
+  // edge traces (-2 x y J^{1,1}_{l-2}(eta) ...) as src/Discretize.cpp:2030-2041
+  std::vector<double> bf((size_t)q2 * ndf), ea((size_t)q2 * nde), eb((size_t)q2 * nde);
+  std::vector<long double> val(S.K);
+  for (int k = 0; k < q2; ++k) {
+    double x = sp->p2[3 * k], y = sp->p2[3 * k + 1];
+    S.ref.eval(x, y, 0.0L, val.data(), nullptr);
+    for (int l1 = 2; l1 <= M; ++l1)
+      for (int l2 = 1; l1 + l2 <= M; ++l2)
+        bf[(size_t)k * ndf + (l1 + l2 - 3) * (l1 + l2 - 2) / 2 + l2 - 1] = (double)val[S.ref.idx.number(l1, l2, 0)];
+    double xi = 2 * x / (1 - y) - 1, eta = 2 * y - 1;
+    for (int l = 2; l <= M; ++l) {
+      ea[(size_t)k * nde + l - 2] = std::pow(1 - y, l - 2) * jacobi(1, 1, l - 2, xi);
+      eb[(size_t)k * nde + l - 2] = jacobi(1, 1, l - 2, eta);
+    }
+  }
+  std::vector<double> b(S.ndof);
+  HIPCHK(hipMemcpyAsync(b.data(), b_dev, S.ndof * 8, hipMemcpyDeviceToHost, sp->st()));
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  for (int f = 0; f < S.topo.nf; ++f) {
+    if (S.mark[2][f] != flag) continue;
+    const double* p0 = &S.xyz[3 * (size_t)S.face_node[f][0]];
+    const double* p1 = &S.xyz[3 * (size_t)S.face_node[f][1]];
+    const double* p2 = &S.xyz[3 * (size_t)S.face_node[f][2]];
+    double u[3], v[3];
+    for (int c = 0; c < 3; ++c) { u[c] = p1[c] - p0[c]; v[c] = p2[c] - p0[c]; }
+    double cx = u[1] * v[2] - u[2] * v[1], cy = u[0] * v[2] - u[2] * v[0], cz = u[0] * v[1] - u[1] * v[0];
+    double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+    double cv[3] = {0, 0, 0};
+    std::vector<double> ce(3 * (size_t)nde, 0.0), cf(ndf, 0.0);
+    for (int k = 0; k < q2; ++k) {
+      const double* bq = &sp->p2[3 * (size_t)k];
+      double wg = sp->w2[k] * gq[(size_t)f * q2 + k] * area;
+      for (int j = 0; j < 3; ++j) cv[j] += wg * bq[(j + 2) % 3];
+      double x = bq[0], y = bq[1], r = bq[2];
+      for (int dd = 0; dd < nde; ++dd) {
+        ce[dd] += wg * -2 * x * y * eb[(size_t)k * nde + dd];
+        ce[nde + dd] += wg * -2 * y * r * eb[(size_t)k * nde + dd];
+        ce[2 * nde + dd] += wg * -2 * x * r * ea[(size_t)k * nde + dd];
+      }
+      for (int dd = 0; dd < ndf; ++dd) cf[dd] += wg * bf[(size_t)k * ndf + dd];
+    }
+    for (int j = 0; j < 3; ++j) b[S.geo_dof(0, S.face_node[f][j])] += cv[j];
+    for (int ie = 0; ie < 3; ++ie) {
+      bool rev = (S.face_rev[f] >> ie) & 1;
+      int64_t st0 = S.geo_dof(1, S.face_edges[f][ie]);
+      for (int dd = 0; dd < nde; ++dd) b[st0 + dd] += (rev && (dd % 2 == 1)) ? -ce[ie * nde + dd] : ce[ie * nde + dd];
+    }
+    int64_t fs = S.geo_dof(2, f);
+    for (int dd = 0; dd < ndf; ++dd) b[fs + dd] += cf[dd];
+  }
+  HIPCHK(hipMemcpyAsync(b_dev, b.data(), S.ndof * 8, hipMemcpyHostToDevice, sp->st()));
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
+static void apply_dev(tsem_space* sp, double stiff, double mass, int constrained, const double* x, double* y,
+                      double* partial) {
+  OpCoef op;
+  op.stiff = stiff; op.mass = mass; op.constrained = constrained;
+  launch_apply<double>(sp->d, op, x, y, sp->yl64.as<double>(), partial, sp->st());
+  HIPCHK(hipGetLastError());
+}
+
+int tsem_dirichlet(tsem_space* sp, const double* uv, const double* ue, const double* uf, int flag, double* b_dev,
+                   double* bval) {
+  API_BEGIN
+  if (!sp || !uv || !ue || !uf || !b_dev) throw std::invalid_argument("null argument");
+  Space& S = sp->h;
+  if (S.mark[0].empty()) { g_err = "boundary marks not set"; return TSEM_ESTATE; }
+  int M = S.M, q1 = sp->nq1, q2 = sp->nq2, nde = S.ndg[1], ndf = S.ndg[2];
+  hipStream_t st = sp->st();
+  std::vector<double> b(S.ndof);
+  HIPCHK(hipMemcpyAsync(b.data(), b_dev, S.ndof * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const std::vector<double>& dg = sp->diag;
+  // interpolation bases: J^{1,1}_{l-2} on edges; on faces
+  // (1-eta)^{l1-2} J^{1,1}_{l1-2}(xi) J^{2l1-1,1}_{l2-1}(eta) and the edge
+  // traces used to remove the edge part (src/BuildSpace.cpp:95-147)
+  std::vector<double> ie1((size_t)q1 * nde), if2((size_t)q2 * std::max(ndf, 1)), ea((size_t)q2 * nde), eb((size_t)q2 * nde);
+  for (int k = 0; k < q1; ++k)
+    for (int l = 2; l <= M; ++l) ie1[(size_t)k * nde + l - 2] = jacobi(1, 1, l - 2, 2 * sp->t1[k] - 1);
+  for (int k = 0; k < q2; ++k) {
+    double x = sp->p2[3 * k], y = sp->p2[3 * k + 1];
+    double xi = 2 * x / (1 - y) - 1, eta = 2 * y - 1;
+    for (int l1 = 2; l1 <= M; ++l1)
+      for (int l2 = 1; l1 + l2 <= M; ++l2)
+        if2[(size_t)k * ndf + (l1 + l2 - 3) * (l1 + l2 - 2) / 2 + l2 - 1] =
+            std::pow(1 - eta, l1 - 2) * jacobi(1, 1, l1 - 2, xi) * jacobi(2 * l1 - 1, 1, l2 - 1, eta);
+    for (int l = 2; l <= M; ++l) {
+      ea[(size_t)k * nde + l - 2] = std::pow(1 - y, l - 2) * jacobi(1, 1, l - 2, xi);
+      eb[(size_t)k * nde + l - 2] = jacobi(1, 1, l - 2, eta);
+    }
+  }
+  std::vector<double> vb(S.ndof, 0.0);
+  std::vector<uint8_t> newmask(S.ndof, 0);
+  if (bval) for (int64_t i = 0; i < S.ndof; ++i) bval[i] = NAN;
+  for (int dim = 0; dim <= 2; ++dim) {
+    int ng = dim == 0 ? S.topo.nv : dim == 1 ? S.topo.ne : S.topo.nf;
+    for (int g = 0; g < ng; ++g) {
+      if (S.mark[dim][g] != flag) continue;
+      int64_t row0 = S.geo_dof(dim, g);
+      std::vector<double> val(S.ndg[dim], 0.0);
+      if (dim == 0) val[0] = uv[g];
+      if (dim == 1) {
+        int64_t ls = S.geo_dof(0, S.edge_node[g][0]), le = S.geo_dof(0, S.edge_node[g][1]);
+        double cs = b[ls] / dg[ls], ce = b[le] / dg[le];
+        for (int l = 2; l <= M; ++l) {
+          double c = 0;
+          for (int k = 0; k < q1; ++k)
+            c += sp->w1[k] * (ue[(size_t)g * q1 + k] - cs * (1 - sp->t1[k]) - ce * sp->t1[k]) * ie1[(size_t)k * nde + l - 2];
+          val[l - 2] = c * -l * (2 * l - 1) / (2 * (l - 1));
+        }
+      }
+      if (dim == 2) {
+        double cvt[3];
+        for (int j = 0; j < 3; ++j) { int64_t p = S.geo_dof(0, S.face_node[g][j]); cvt[j] = b[p] / dg[p]; }
+        std::vector<double> ved(3 * (size_t)nde);
+        for (int ie = 0; ie < 3; ++ie) {
+          int64_t es = S.geo_dof(1, S.face_edges[g][ie]);
+          bool rev = (S.face_rev[g] >> ie) & 1;
+          for (int l = 2; l <= M; ++l) {
+            double v = b[es + l - 2] / dg[es + l - 2];
+            ved[ie * nde + l - 2] = (rev && l % 2 == 1) ? -v : v;
+          }
+        }
+        for (int l1 = 2; l1 <= M; ++l1)
+          for (int l2 = 1; l1 + l2 <= M; ++l2) {
+            int id = (l1 + l2 - 3) * (l1 + l2 - 2) / 2 + l2 - 1;
+            double c = 0;
+            for (int k = 0; k < q2; ++k) {
+              const double* bq = &sp->p2[3 * (size_t)k];
+              double x = bq[0], y = bq[1], r = bq[2];
+              double cp = uf[(size_t)g * q2 + k];
+              for (int j = 0; j < 3; ++j) cp -= cvt[j] * bq[(j + 2) % 3];
+              for (int l = 2; l <= M; ++l)
+                cp += 2 * (x * y * ved[l - 2] * eb[(size_t)k * nde + l - 2] + y * r * ved[nde + l - 2] * eb[(size_t)k * nde + l - 2] +
+                           x * r * ved[2 * nde + l - 2] * ea[(size_t)k * nde + l - 2]);
+              c += sp->w2[k] * cp * if2[(size_t)k * ndf + id];
+            }
+            val[id] = c * -l1 * (2 * l1 - 1) * (2 * l1 + 2 * l2 - 1) / (std::pow(2, l1) * (l1 - 1));
+          }
+      }
+      for (int i = 0; i < S.ndg[dim]; ++i) {
+        int64_t row = row0 + i;
+        b[row] = dg[row] * val[i];
+        vb[row] = val[i];
+        newmask[row] = 1;
+        if (bval) bval[row] = val[i];
+      }
+    }
+  }
+  // free rows: b -= A ext(val), with the elimination imposed so far
+  DevBuf tmpv, tmpw;
+  upload(tmpv, vb, st);
+  tmpw.alloc(S.ndof * 8);
+  apply_dev(sp, 1.0, 0.0, 1, tmpv.as<double>(), tmpw.as<double>(), nullptr);
+  std::vector<double> w(S.ndof);
+  HIPCHK(hipMemcpyAsync(w.data(), tmpw.p, S.ndof * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int64_t i = 0; i < S.ndof; ++i)
+    if (!newmask[i] && !sp->cmask[i]) b[i] -= w[i];
+  for (int64_t i = 0; i < S.ndof; ++i) sp->cmask[i] |= newmask[i];
+  upload(sp->cmask_d, sp->cmask, st);
+  HIPCHK(hipMemcpyAsync(b_dev, b.data(), S.ndof * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  sp->pmg.reset();
+  API_END
+}
+
+int tsem_zero_bc(tsem_space* sp, int flag) {
+  API_BEGIN
+  if (!sp) throw std::invalid_argument("null space");
+  Space& S = sp->h;
+  if (S.mark[0].empty()) { g_err = "boundary marks not set"; return TSEM_ESTATE; }
+  for (int dim = 0; dim <= 2; ++dim) {
+    int ng = dim == 0 ? S.topo.nv : dim == 1 ? S.topo.ne : S.topo.nf;
+    for (int g = 0; g < ng; ++g)
+      if (S.mark[dim][g] == flag)
+        for (int i = 0; i < S.ndg[dim]; ++i) sp->cmask[S.geo_dof(dim, g) + i] = 1;
+  }
+  upload(sp->cmask_d, sp->cmask, sp->st());
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  sp->pmg.reset();
+  API_END
+}
+
+int tsem_apply(tsem_space* sp, double stiff, double mass, int constrained, int prec, const void* x, void* y) {
+  API_BEGIN
+  if (!sp || !x || !y) throw std::invalid_argument("null argument");
+  OpCoef op;
+  op.stiff = stiff; op.mass = mass; op.constrained = constrained;
+  if (prec == 64) launch_apply<double>(sp->d, op, (const double*)x, (double*)y, sp->yl64.as<double>(), nullptr, sp->st());
+  else if (prec == 32) launch_apply<float>(sp->d, op, (const float*)x, (float*)y, sp->yl32.as<float>(), nullptr, sp->st());
+  else throw std::invalid_argument("prec must be 64 or 32");
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
+int tsem_gather(tsem_space* sp, const double* x, double* xl) {
+  API_BEGIN
+  if (!sp || !x || !xl) throw std::invalid_argument("null argument");
+  launch_gather(sp->d, x, xl, 0, sp->st());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
+int tsem_scatter(tsem_space* sp, const double* yl, double* y) {
+  API_BEGIN
+  if (!sp || !yl || !y) throw std::invalid_argument("null argument");
+  launch_scatter<double>(sp->d, yl, y, sp->st());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
+int tsem_diagonal(tsem_space* sp, double* dd) {
+  API_BEGIN
+  if (!sp || !dd) throw std::invalid_argument("null argument");
+  HIPCHK(hipMemcpyAsync(dd, sp->diag_d.p, sp->h.ndof * 8, hipMemcpyDeviceToDevice, sp->st()));
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- solvers ------
+namespace {
+
+// Preconditioned CG, reference formulation (src/TSEMSolver.cpp:1775-1930):
+// r = A x - b, y = P r, p = -y, alpha = r.y / p.Ap, x += alpha p,
+// r += alpha Ap, beta = r.y_new / r.y, p = beta p - y.
+// stop: criterion 0 -> |r|_1 < tol, 1 -> |r|_2 / bnorm < tol (recursive r).
+template <typename T>
+int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double bnorm, int max_iter, int precond,
+             int smooth, int check_every, int* iters, double* l1_out, double* l2_out) {
+  const DevSpace& d = sp->d;
+  int n = d.ndof;
+  hipStream_t st = sp->st();
+  DevBuf* V = (sizeof(T) == 8) ? sp->v64 : sp->v32;
+  for (int k = 0; k < 4; ++k) V[k].alloc((size_t)n * sizeof(T));
+  T* r = V[0].as<T>(); T* z = V[1].as<T>(); T* p = V[2].as<T>(); T* q = V[3].as<T>();
+  T* yl = (sizeof(T) == 8) ? (T*)sp->yl64.p : (T*)sp->yl32.p;
+  CgScalars* sc = sp->scal.as<CgScalars>();
+  double* part = sp->partial.as<double>();
+  int nb = reduce_blocks(n);
+  OpCoef op;
+  op.constrained = 1;
+  // r = A x - b
+  launch_apply<T>(d, op, x, r, yl, nullptr, st);
+  launch_axpby<T>(n, T(-1), b, T(1), r, st);
+  // initial preconditioned residual and scalars: the update kernel with
+  // alpha = 0 computes z = D^-1 r (Jacobi) and the partial sums
+  CgScalars zero{};
+  HIPCHK(hipMemcpyAsync(sc, &zero, sizeof(CgScalars), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(p, 0, (size_t)n * sizeof(T), st));
+  HIPCHK(hipMemsetAsync(q, 0, (size_t)n * sizeof(T), st));
+  auto update_and_precondition = [&]() {
+    launch_cg_update<T>(d, n, x, r, precond == 4 ? nullptr : z, p, q, sc, part, precond == 4 ? 0 : 1, st);
+    launch_reduce_final(part, nb, 3, sc->tmp, st);
+    if (precond == 4) {
+      sp->pmg->vcycle<T>(r, z, smooth, st);
+      launch_dot<T>(n, r, z, part, st);
+      launch_reduce_final(part, nb, 1, sc->tmp, st);
+    }
+    launch_scalar_beta(sc, st);
+  };
+  update_and_precondition();
+  // p = -z
+  launch_axpby<T>(n, T(-1), z, T(0), p, st);
+  CgScalars hs{};
+  HIPCHK(hipMemcpyAsync(&hs, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  auto resid = [&](const CgScalars& s) {
+    return criterion == 0 ? s.l1 : std::sqrt(s.l2) / bnorm;
+  };
+  double res = resid(hs);
+  int it = 0;
+  while (res >= tol && it <= max_iter) {
+    int nrun = std::max(1, check_every);
+    for (int k = 0; k < nrun && it <= max_iter; ++k) {
+      launch_apply<T>(d, op, p, q, yl, part, st);
+      launch_reduce_final(part, reduce_blocks(n), 1, &sc->pq, st);
+      launch_scalar_alpha(sc, st);
+      update_and_precondition();
+      launch_p_update<T>(n, p, z, sc, st);
+      ++it;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&hs, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    res = resid(hs);
+    if (!std::isfinite(res)) break;
+  }
+  *iters = it;
+  *l1_out = hs.l1;
+  *l2_out = std::sqrt(hs.l2);
+  return (res < tol) ? TSEM_OK : TSEM_ENOCONV;
+}
+
+double true_relres(tsem_space* sp, const double* b, const double* x, double* r, double* bn) {
+  int n = sp->d.ndof;
+  hipStream_t st = sp->st();
+  DevBuf& t = sp->v64[5];
+  t.alloc((size_t)n * 8);
+  apply_dev(sp, 1.0, 0.0, 1, x, t.as<double>(), nullptr);
+  double* part = sp->partial.as<double>();
+  int nb = reduce_blocks(n);
+  launch_residual(sp->d, n, b, t.as<double>(), r, part, st);
+  CgScalars* sc = sp->scal.as<CgScalars>();
+  launch_reduce_final(part, nb, 2, sc->tmp + 4, st);
+  double h[2];
+  HIPCHK(hipMemcpyAsync(h, sc->tmp + 4, 16, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (bn) *bn = std::sqrt(h[1]);
+  return std::sqrt(h[0]) / std::sqrt(h[1]);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tsem_pcg(tsem_space* sp, const double* b, double* x, const tsem_pcg_opts* o, tsem_pcg_stats* stats) {
+  API_BEGIN
+  if (!sp || !b || !x || !o) throw std::invalid_argument("null argument");
+  if (o->precond != 1 && o->precond != 4) throw std::invalid_argument("precond must be 1 (Jacobi) or 4 (p-MG)");
+  if (!(o->tol > 0)) throw std::invalid_argument("tol must be positive");
+  int n = sp->d.ndof;
+  hipStream_t st = sp->st();
+  if (o->precond == 4 && !sp->pmg) sp->pmg.reset(new Pmg(sp->h, sp->d, sp->cmask, st));
+  auto t0 = std::chrono::steady_clock::now();
+  tsem_pcg_stats s{};
+  int rc = TSEM_OK;
+  sp->v64[4].alloc((size_t)n * 8);
+  double* r64 = sp->v64[4].as<double>();
+  double bnorm = 0;
+  if (!o->mixed) {
+    double rr = true_relres(sp, b, x, r64, &bnorm);
+    (void)rr;
+    if (bnorm == 0) bnorm = 1;
+    int it = 0;
+    double l1 = 0, l2 = 0;
+    rc = pcg_core<double>(sp, b, x, o->tol, o->criterion, bnorm, o->max_iter, o->precond, o->smooth_steps,
+                          o->check_every, &it, &l1, &l2);
+    s.iterations = it;
+    s.outer = 1;
+    s.l1 = l1;
+    s.relres = true_relres(sp, b, x, r64, nullptr);
+  } else {
+    // FP64 iterative refinement around FP32 inner PCG
+    for (int k = 0; k < 4; ++k) sp->v32[k].alloc((size_t)n * 4);
+    sp->v32[4].alloc((size_t)n * 4);
+    sp->v32[5].alloc((size_t)n * 4);
+    float* r32 = sp->v32[4].as<float>();
+    float* d32 = sp->v32[5].as<float>();
+    double rel = true_relres(sp, b, x, r64, &bnorm);
+    if (bnorm == 0) bnorm = 1;
+    int outer = 0, total = 0;
+    rc = TSEM_OK;
+    while (rel > o->tol) {
+      if (outer >= o->max_iter) { rc = TSEM_ENOCONV; break; }
+      double rn = rel * bnorm;
+      launch_cast_f64_f32(n, r64, 1.0 / rn, r32, st);
+      HIPCHK(hipMemsetAsync(d32, 0, (size_t)n * 4, st));
+      int it = 0;
+      double l1 = 0, l2 = 0;
+      pcg_core<float>(sp, r32, d32, o->inner_tol, 1, 1.0, o->inner_max, o->precond, o->smooth_steps,
+                      o->check_every, &it, &l1, &l2);
+      total += it;
+      launch_add_f32_f64(n, d32, rn, x, st);
+      rel = true_relres(sp, b, x, r64, nullptr);
+      ++outer;
+      if (!std::isfinite(rel)) { rc = TSEM_ENOCONV; break; }
+    }
+    s.iterations = total;
+    s.outer = outer;
+    s.relres = rel;
+    s.l1 = 0;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) *stats = s;
+  if (rc != TSEM_OK) { g_err = "PCG did not reach the tolerance"; return rc; }
+  API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,474 @@
+// HIP kernels of the TSEM operator path for gfx950.
+//
+//  gather   : element-local coefficients from global DOFs through the compact
+//             element descriptors and the face re-orientation table
+//             (TSEMLib transform_*_global2local, src/BuildSpace.cpp:1021-1101);
+//  apply    : y_e = sum_ab Ghat_ab D_a^T D_b x_e  (+ mass term), the element
+//             operator sum_k w_k S_k of src/Discretize.cpp:1352-1393 written
+//             through an orthonormal factorization of the reference stiffness;
+//  scatter  : owner-computes sum over the transposed gather map, fixed order,
+//             no atomics (bit-identical run to run);
+//  CG/vector: fused PCG updates with deterministic two-stage reductions.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace tsem {
+
+constexpr int BLK = 256;
+
+// ---------------------------------------------------------------- gather ----
+template <typename T>
+__device__ inline T ldx(const T* __restrict__ x, const uint8_t* __restrict__ cmask, int g, int constrained) {
+  if (constrained && cmask[g]) return T(0);
+  return x[g];
+}
+
+// one local coefficient; FP64 sums are kept free of fused multiply-adds so the
+// result is bit-identical to a plain sequential evaluation
+template <typename T>
+__device__ inline T gather_slot(const DevSpace& S, int e, int i, const T* __restrict__ x, int constrained) {
+  const ElemDesc& D = S.desc[e];
+  int code = S.slot[i];
+  int kind = code & 3, geo = (code >> 2) & 7, sub = code >> 5;
+  if (kind == 0) return ldx(x, S.cmask, D.v[geo], constrained);
+  if (kind == 1) {
+    int ed = D.e[geo];
+    T v = ldx(x, S.cmask, (ed & 0x7fffffff) + sub, constrained);
+    // reversed edge: odd-degree modes (l = sub + 2 odd) change sign
+    return (ed < 0 && (sub & 1)) ? -v : v;
+  }
+  if (kind == 3) return ldx(x, S.cmask, D.t + sub, constrained);
+  int fo = D.fo[geo];
+  int var = fo & 7;
+  int row = var * S.nface + sub;
+  T acc = T(0);
+  for (int t = S.ft_ptr[row]; t < S.ft_ptr[row + 1]; ++t) {
+    int key = S.ft_key[t];
+    int ie = key & 3, idx = key >> 2;
+    T w = (T)S.ft_w[t];
+    int g;
+    if (ie < 3) {
+      int le = (fo >> (3 + 3 * ie)) & 7;
+      if (((fo >> (12 + ie)) & 1) && (idx & 1)) w = -w;
+      g = (D.e[le] & 0x7fffffff) + idx;
+    } else {
+      g = D.f[geo] + idx;
+    }
+    T xv = ldx(x, S.cmask, g, constrained);
+    if constexpr (sizeof(T) == 8) acc = __dadd_rn(acc, __dmul_rn(xv, w));
+    else acc += xv * w;
+  }
+  return acc;
+}
+
+__global__ void __launch_bounds__(BLK) k_gather(DevSpace S, const double* __restrict__ x, double* __restrict__ xl,
+                                                int constrained) {
+  long s = (long)blockIdx.x * BLK + threadIdx.x;
+  long n = (long)S.nt * S.K;
+  if (s >= n) return;
+  int e = (int)(s / S.K), i = (int)(s % S.K);
+  xl[s] = gather_slot<double>(S, e, i, x, constrained);
+}
+
+void launch_gather(const DevSpace& S, const double* x, double* xl, int constrained, hipStream_t st) {
+  long n = (long)S.nt * S.K;
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + BLK - 1) / BLK)), dim3(BLK), 0, st, S, x, xl, constrained);
+}
+
+// ----------------------------------------------------------------- apply ----
+template <typename T> struct Fac;
+template <> struct Fac<double> {
+  static __device__ const double* dT(const DevSpace& S) { return S.dT; }
+  static __device__ const double* dN(const DevSpace& S) { return S.dN; }
+  static __device__ const double* mT(const DevSpace& S) { return S.mT; }
+  static __device__ const double* mN(const DevSpace& S) { return S.mN; }
+  static __device__ double g(const DevSpace& S, int e, int k) { return S.ghat[6 * e + k]; }
+  static __device__ double v6(const DevSpace& S, int e) { return S.sixv[e]; }
+};
+template <> struct Fac<float> {
+  static __device__ const float* dT(const DevSpace& S) { return S.dT32; }
+  static __device__ const float* dN(const DevSpace& S) { return S.dN32; }
+  static __device__ const float* mT(const DevSpace& S) { return S.mT32; }
+  static __device__ const float* mN(const DevSpace& S) { return S.mN32; }
+  static __device__ float g(const DevSpace& S, int e, int k) { return S.ghat32[6 * e + k]; }
+  static __device__ float v6(const DevSpace& S, int e) { return S.sixv32[e]; }
+};
+
+// NE elements per block; LDS: X[NE][K] | U[NE][3Kd] | Wm[NE][K] (mass)
+template <typename T, int NE>
+__global__ void __launch_bounds__(BLK) k_apply(DevSpace S, OpCoef op, const T* __restrict__ x, T* __restrict__ yl) {
+  extern __shared__ unsigned char smem_raw[];
+  T* X = reinterpret_cast<T*>(smem_raw);
+  const int K = S.K, Kd = S.Kd, R = 3 * Kd;
+  const int Kl = op.Kl > 0 ? op.Kl : K;
+  const int Kdl = op.Kdl > 0 ? op.Kdl : Kd;
+  T* U = X + NE * K;
+  T* W = U + NE * R;
+  const int e0 = blockIdx.x * NE;
+  const int tid = threadIdx.x;
+  for (int s = tid; s < NE * K; s += BLK) {
+    int le = s / K, i = s % K, e = e0 + le;
+    X[s] = (e < S.nt && i < Kl) ? gather_slot<T>(S, e, i, x, op.constrained) : T(0);
+  }
+  __syncthreads();
+  const T* dT = Fac<T>::dT(S);
+  const T* dN = Fac<T>::dN(S);
+  if (op.stiff != 0.0) {
+    // U[e][a Kd + m] = sum_i D_a[m][i] X[e][i]
+    for (int r = tid; r < R; r += BLK) {
+      int m = r % Kd;
+      T acc[NE];
+#pragma unroll
+      for (int le = 0; le < NE; ++le) acc[le] = T(0);
+      if (m < Kdl) {
+        for (int i = 0; i < Kl; ++i) {
+          T d = dT[(size_t)i * R + r];
+#pragma unroll
+          for (int le = 0; le < NE; ++le) acc[le] += d * X[le * K + i];
+        }
+      }
+#pragma unroll
+      for (int le = 0; le < NE; ++le) U[le * R + r] = acc[le];
+    }
+    __syncthreads();
+    // V_a = sum_b Ghat_ab U_b
+    for (int s = tid; s < NE * Kd; s += BLK) {
+      int le = s / Kd, m = s % Kd, e = e0 + le;
+      if (e >= S.nt) continue;
+      T g0 = Fac<T>::g(S, e, 0), g1 = Fac<T>::g(S, e, 1), g2 = Fac<T>::g(S, e, 2);
+      T g3 = Fac<T>::g(S, e, 3), g4 = Fac<T>::g(S, e, 4), g5 = Fac<T>::g(S, e, 5);
+      T c = (T)op.stiff;
+      T u0 = U[le * R + m], u1 = U[le * R + Kd + m], u2 = U[le * R + 2 * Kd + m];
+      U[le * R + m] = c * (g0 * u0 + g3 * u1 + g4 * u2);
+      U[le * R + Kd + m] = c * (g3 * u0 + g1 * u1 + g5 * u2);
+      U[le * R + 2 * Kd + m] = c * (g4 * u0 + g5 * u1 + g2 * u2);
+    }
+    __syncthreads();
+  }
+  const bool has_mass = op.mass != 0.0;
+  if (has_mass) {
+    const T* mT = Fac<T>::mT(S);
+    for (int r = tid; r < K; r += BLK) {
+      T acc[NE];
+#pragma unroll
+      for (int le = 0; le < NE; ++le) acc[le] = T(0);
+      if (r < Kl)
+        for (int i = 0; i < Kl; ++i) {
+          T d = mT[(size_t)i * K + r];
+#pragma unroll
+          for (int le = 0; le < NE; ++le) acc[le] += d * X[le * K + i];
+        }
+#pragma unroll
+      for (int le = 0; le < NE; ++le) {
+        int e = e0 + le;
+        W[le * K + r] = (e < S.nt) ? acc[le] * (T)op.mass * Fac<T>::v6(S, e) : T(0);
+      }
+    }
+    __syncthreads();
+  }
+  const T* mN = Fac<T>::mN(S);
+  for (int s = tid; s < NE * K; s += BLK) {
+    int le = s / K, i = s % K, e = e0 + le;
+    if (e >= S.nt) continue;
+    T acc = T(0);
+    if (i < Kl) {
+      if (op.stiff != 0.0)
+        for (int a = 0; a < 3; ++a)
+          for (int m = 0; m < Kdl; ++m) acc += dN[(size_t)(a * Kd + m) * K + i] * U[le * R + a * Kd + m];
+      if (has_mass)
+        for (int m = 0; m < Kl; ++m) acc += mN[(size_t)m * K + i] * W[le * K + m];
+    }
+    yl[(size_t)e * K + i] = acc;
+  }
+}
+
+// ---------------------------------------------------------------- scatter ----
+// y[g] = sum over the transposed map (or diag * x on eliminated DOFs); the
+// optional partial sums of x.y feed the CG denominator.
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict__ yl, T* __restrict__ y,
+                                                 const T* __restrict__ xin, int constrained,
+                                                 double* __restrict__ partial) {
+  __shared__ double red[BLK];
+  int g = blockIdx.x * BLK + threadIdx.x;
+  double d = 0.0;
+  if (g < S.ndof) {
+    T acc = T(0);
+    if (constrained && S.cmask[g]) {
+      acc = (T)S.diag[g] * xin[g];
+    } else {
+      for (int t = S.sc_ptr[g]; t < S.sc_ptr[g + 1]; ++t) {
+        int code = S.sc_code[t];
+        T v = yl[code & 0x3fffffff];
+        if (code < 0) {
+          if constexpr (sizeof(T) == 8) v = __dmul_rn(v, S.sc_w[t]);
+          else v = v * (T)S.sc_w[t];
+        } else if (code & 0x40000000) {
+          v = -v;
+        }
+        if constexpr (sizeof(T) == 8) acc = __dadd_rn(acc, v);
+        else acc += v;
+      }
+    }
+    y[g] = acc;
+    if (partial) d = (double)xin[g] * (double)acc;
+  }
+  if (!partial) return;
+  red[threadIdx.x] = d;
+  __syncthreads();
+  for (int s = BLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+template <typename T>
+static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, double* partial, hipStream_t st) {
+  constexpr int NE = 8;
+  size_t lds = (size_t)NE * (2 * S.K + 3 * S.Kd) * sizeof(T);
+  int nb = (S.nt + NE - 1) / NE;
+  hipLaunchKernelGGL((k_apply<T, NE>), dim3(nb), dim3(BLK), lds, st, S, op, x, yl);
+  int ng = (S.ndof + BLK - 1) / BLK;
+  hipLaunchKernelGGL(k_scatter<T>, dim3(ng), dim3(BLK), 0, st, S, yl, y, x, op.constrained, partial);
+}
+
+template <>
+void launch_apply<double>(const DevSpace& S, const OpCoef& op, const double* x, double* y, double* yl,
+                          double* partial, hipStream_t st) {
+  apply_impl<double>(S, op, x, y, yl, partial, st);
+}
+template <>
+void launch_apply<float>(const DevSpace& S, const OpCoef& op, const float* x, float* y, float* yl, double* partial,
+                         hipStream_t st) {
+  apply_impl<float>(S, op, x, y, yl, partial, st);
+}
+
+template <typename T>
+void launch_scatter(const DevSpace& S, const T* yl, T* y, hipStream_t st) {
+  int ng = (S.ndof + BLK - 1) / BLK;
+  hipLaunchKernelGGL(k_scatter<T>, dim3(ng), dim3(BLK), 0, st, S, yl, y, (const T*)nullptr, 0, (double*)nullptr);
+}
+template void launch_scatter<double>(const DevSpace&, const double*, double*, hipStream_t);
+template void launch_scatter<float>(const DevSpace&, const float*, float*, hipStream_t);
+
+// ------------------------------------------------------------- reductions ----
+int reduce_blocks(int n) { return (n + BLK - 1) / BLK; }
+
+// out[v] = sum_b partial[v * nb + b], fixed tree
+__global__ void __launch_bounds__(BLK) k_reduce_final(const double* __restrict__ partial, int nb, int nvals,
+                                                      double* __restrict__ out) {
+  __shared__ double red[BLK];
+  for (int v = 0; v < nvals; ++v) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += BLK) s += partial[(size_t)v * nb + b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = BLK / 2; k > 0; k >>= 1) {
+      if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[v] = red[0];
+    __syncthreads();
+  }
+}
+
+void launch_reduce_final(const double* partial, int nb, int nvals, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(BLK), 0, st, partial, nb, nvals, out);
+}
+
+// --------------------------------------------------------------- CG steps ----
+__global__ void k_alpha(CgScalars* sc) { sc->alpha = sc->rz / sc->pq; }
+// tmp[0..2] = r.z, |r|_1, |r|_2^2 of the new residual
+__global__ void k_beta(CgScalars* sc) {
+  sc->rz_new = sc->tmp[0];
+  sc->beta = sc->rz_new / sc->rz;
+  sc->rz = sc->rz_new;
+  sc->l1 = sc->tmp[1];
+  sc->l2 = sc->tmp[2];
+}
+void launch_scalar_alpha(CgScalars* sc, hipStream_t st) { hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1), 0, st, sc); }
+void launch_scalar_beta(CgScalars* sc, hipStream_t st) { hipLaunchKernelGGL(k_beta, dim3(1), dim3(1), 0, st, sc); }
+
+// x += alpha p ; r += alpha q ; z = r / diag (Jacobi) or z = r ;
+// partial sums of r.z, |r|_1, |r|_2^2  (src/TSEMSolver.cpp:1846-1897)
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_cg_update(DevSpace S, int n, T* __restrict__ x, T* __restrict__ r,
+                                                   T* __restrict__ z, const T* __restrict__ p,
+                                                   const T* __restrict__ q, const CgScalars* __restrict__ sc,
+                                                   double* __restrict__ partial, int jac) {
+  __shared__ double red[3][BLK];
+  int i = blockIdx.x * BLK + threadIdx.x;
+  double a = 0, b = 0, c = 0;
+  if (i < n) {
+    T alpha = (T)sc->alpha;
+    x[i] += alpha * p[i];
+    T ri = r[i] + alpha * q[i];
+    r[i] = ri;
+    T zi;
+    if (jac) {
+      if constexpr (sizeof(T) == 8) zi = ri * S.invdiag[i];
+      else zi = ri * S.invdiag32[i];
+    } else {
+      zi = ri;
+    }
+    if (z) z[i] = zi;
+    a = (double)ri * (double)zi;
+    b = fabs((double)ri);
+    c = (double)ri * (double)ri;
+  }
+  red[0][threadIdx.x] = a; red[1][threadIdx.x] = b; red[2][threadIdx.x] = c;
+  __syncthreads();
+  for (int s = BLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+      red[2][threadIdx.x] += red[2][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = red[0][0];
+    partial[gridDim.x + blockIdx.x] = red[1][0];
+    partial[2 * gridDim.x + blockIdx.x] = red[2][0];
+  }
+}
+
+template <typename T>
+void launch_cg_update(const DevSpace& S, int n, T* x, T* r, T* z, const T* p, const T* q, CgScalars* sc,
+                      double* partial, int jac, hipStream_t st) {
+  int nb = reduce_blocks(n);
+  hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(BLK), 0, st, S, n, x, r, z, p, q, sc, partial, jac);
+}
+template void launch_cg_update<double>(const DevSpace&, int, double*, double*, double*, const double*, const double*,
+                                       CgScalars*, double*, int, hipStream_t);
+template void launch_cg_update<float>(const DevSpace&, int, float*, float*, float*, const float*, const float*,
+                                      CgScalars*, double*, int, hipStream_t);
+
+// p = beta p - z
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_p_update(int n, T* __restrict__ p, const T* __restrict__ z,
+                                                  const CgScalars* __restrict__ sc) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i < n) p[i] = (T)sc->beta * p[i] - z[i];
+}
+template <typename T>
+void launch_p_update(int n, T* p, const T* z, CgScalars* sc, hipStream_t st) {
+  hipLaunchKernelGGL(k_p_update<T>, dim3(reduce_blocks(n)), dim3(BLK), 0, st, n, p, z, sc);
+}
+template void launch_p_update<double>(int, double*, const double*, CgScalars*, hipStream_t);
+template void launch_p_update<float>(int, float*, const float*, CgScalars*, hipStream_t);
+
+template <typename T>
+__global__ void k_axpby(int n, T a, const T* __restrict__ x, T b, T* __restrict__ y) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+template <typename T>
+void launch_axpby(int n, T a, const T* x, T b, T* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpby<T>, dim3(reduce_blocks(n)), dim3(BLK), 0, st, n, a, x, b, y);
+}
+template void launch_axpby<double>(int, double, const double*, double, double*, hipStream_t);
+template void launch_axpby<float>(int, float, const float*, float, float*, hipStream_t);
+
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_dot(int n, const T* __restrict__ a, const T* __restrict__ b,
+                                             double* __restrict__ partial) {
+  __shared__ double red[BLK];
+  int i = blockIdx.x * BLK + threadIdx.x;
+  red[threadIdx.x] = (i < n) ? (double)a[i] * (double)b[i] : 0.0;
+  __syncthreads();
+  for (int s = BLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+template <typename T>
+void launch_dot(int n, const T* a, const T* b, double* partial, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot<T>, dim3(reduce_blocks(n)), dim3(BLK), 0, st, n, a, b, partial);
+}
+template void launch_dot<double>(int, const double*, const double*, double*, hipStream_t);
+template void launch_dot<float>(int, const float*, const float*, double*, hipStream_t);
+
+__global__ void k_cast(int n, const double* __restrict__ x, double s, float* __restrict__ y) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i < n) y[i] = (float)(x[i] * s);
+}
+void launch_cast_f64_f32(int n, const double* x, double s, float* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_cast, dim3(reduce_blocks(n)), dim3(BLK), 0, st, n, x, s, y);
+}
+__global__ void k_addf(int n, const float* __restrict__ x, double s, double* __restrict__ y) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i < n) y[i] += s * (double)x[i];
+}
+void launch_add_f32_f64(int n, const float* x, double s, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_addf, dim3(reduce_blocks(n)), dim3(BLK), 0, st, n, x, s, y);
+}
+
+// r = b - Ax ; partial |r|_2^2 and |b|_2^2
+__global__ void __launch_bounds__(BLK) k_residual(int n, const double* __restrict__ b, const double* __restrict__ Ax,
+                                                  double* __restrict__ r, double* __restrict__ partial) {
+  __shared__ double red[2][BLK];
+  int i = blockIdx.x * BLK + threadIdx.x;
+  double a = 0, c = 0;
+  if (i < n) {
+    double v = b[i] - Ax[i];
+    r[i] = v;
+    a = v * v;
+    c = b[i] * b[i];
+  }
+  red[0][threadIdx.x] = a; red[1][threadIdx.x] = c;
+  __syncthreads();
+  for (int s = BLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) { red[0][threadIdx.x] += red[0][threadIdx.x + s]; red[1][threadIdx.x] += red[1][threadIdx.x + s]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { partial[blockIdx.x] = red[0][0]; partial[gridDim.x + blockIdx.x] = red[1][0]; }
+}
+void launch_residual(const DevSpace& S, int n, const double* b, const double* Ax, double* r, double* partial,
+                     hipStream_t st) {
+  (void)S;
+  hipLaunchKernelGGL(k_residual, dim3(reduce_blocks(n)), dim3(BLK), 0, st, n, b, Ax, r, partial);
+}
+
+// -------------------------------------------------------------------- RHS ----
+// bl[e][i] = vol_e * sum_q (w_q phi_q[i]) f[e][q]
+__global__ void __launch_bounds__(BLK) k_rhs_local(DevSpace S, int e0, int e1, int nq, const double* __restrict__ wphi,
+                                                   const double* __restrict__ fq, double* __restrict__ bl) {
+  long s = (long)blockIdx.x * BLK + threadIdx.x;
+  long n = (long)(e1 - e0) * S.K;
+  if (s >= n) return;
+  int le = (int)(s / S.K), i = (int)(s % S.K);
+  int e = e0 + le;
+  const double* f = fq + (size_t)le * nq;
+  double acc = 0;
+  for (int q = 0; q < nq; ++q) acc += wphi[(size_t)q * S.K + i] * f[q];
+  bl[(size_t)e * S.K + i] = acc * (S.sixv[e] / 6.0);
+}
+void launch_rhs_local(const DevSpace& S, int e0, int e1, int nq, const double* wphi, const double* fq, double* bl,
+                      hipStream_t st) {
+  long n = (long)(e1 - e0) * S.K;
+  hipLaunchKernelGGL(k_rhs_local, dim3((unsigned)((n + BLK - 1) / BLK)), dim3(BLK), 0, st, S, e0, e1, nq, wphi, fq, bl);
+}
+
+// physical coordinates of reference points on elements [e0, e1)
+__global__ void k_qp_coords(const double* __restrict__ xe, int e0, int e1, int nq, const double* __restrict__ pts,
+                            double* __restrict__ out) {
+  long s = (long)blockIdx.x * BLK + threadIdx.x;
+  long n = (long)(e1 - e0) * nq;
+  if (s >= n) return;
+  int le = (int)(s / nq), q = (int)(s % nq);
+  const double* P = xe + (size_t)(e0 + le) * 12;
+  double x = pts[3 * q], y = pts[3 * q + 1], z = pts[3 * q + 2], l0 = 1 - x - y - z;
+  for (int d = 0; d < 3; ++d) out[s * 3 + d] = l0 * P[d] + x * P[3 + d] + y * P[6 + d] + z * P[9 + d];
+}
+void launch_qp_coords(const DevSpace& S, const double* xe, int e0, int e1, int nq, const double* pts, double* out,
+                      hipStream_t st) {
+  (void)S;
+  long n = (long)(e1 - e0) * nq;
+  hipLaunchKernelGGL(k_qp_coords, dim3((unsigned)((n + BLK - 1) / BLK)), dim3(BLK), 0, st, xe, e0, e1, nq, pts, out);
+}
+
+}  // namespace tsem

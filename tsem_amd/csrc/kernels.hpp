@@ -1,0 +1,84 @@
+// Device-side data of a space and the launchers of the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "space.hpp"
+
+namespace tsem {
+
+// Everything the kernels read about a space (device pointers).
+struct DevSpace {
+  int nt = 0, K = 0, Kd = 0, M = 0, nface = 0;
+  int ndof = 0;
+  const ElemDesc* desc = nullptr;
+  const int32_t* slot = nullptr;     // per local mode: kind | geo << 2 | sub << 5
+  const int32_t* ft_ptr = nullptr;   // face table CSR over (variant, face dof)
+  const int32_t* ft_key = nullptr;   // ie | idx << 2
+  const double* ft_w = nullptr;
+  const double* ghat = nullptr;      // nt x 6
+  const float* ghat32 = nullptr;
+  const double* sixv = nullptr;      // nt
+  const float* sixv32 = nullptr;
+  const double* dT = nullptr;        // K x 3Kd  (derivative factors, transposed)
+  const double* dN = nullptr;        // 3Kd x K
+  const float* dT32 = nullptr;
+  const float* dN32 = nullptr;
+  const double* mT = nullptr;        // K x K (mass factor transposed)
+  const double* mN = nullptr;
+  const float* mT32 = nullptr;
+  const float* mN32 = nullptr;
+  const int32_t* sc_ptr = nullptr;   // ndof + 1
+  const int32_t* sc_code = nullptr;
+  const double* sc_w = nullptr;
+  const double* diag = nullptr;      // assembled operator diagonal
+  const double* invdiag = nullptr;
+  const float* invdiag32 = nullptr;
+  const uint8_t* cmask = nullptr;    // 1 where the DOF is eliminated (Dirichlet)
+};
+
+// operator selection
+struct OpCoef {
+  double stiff = 1.0, mass = 0.0;
+  int constrained = 0;   // honour the Dirichlet elimination
+  int Kl = 0, Kdl = 0;   // leading modes used (p-multigrid levels); 0 = all
+};
+
+// scalar block of the CG loops (device memory)
+struct CgScalars {
+  double pq, rz, rz_new, alpha, beta, l1, l2, b2;
+  double tmp[8];
+};
+
+void launch_gather(const DevSpace& S, const double* x, double* xl, int constrained, hipStream_t st);
+template <typename T>
+void launch_apply(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, double* partial, hipStream_t st);
+template <typename T>
+void launch_scatter(const DevSpace& S, const T* yl, T* y, hipStream_t st);
+
+// deterministic reductions: partial sums per block then a single-block final
+int reduce_blocks(int n);
+void launch_reduce_final(const double* partial, int nb, int nvals, double* out, hipStream_t st);
+
+template <typename T>
+void launch_cg_update(const DevSpace& S, int n, T* x, T* r, T* z, const T* p, const T* q, CgScalars* sc,
+                      double* partial, int precond_jacobi, hipStream_t st);
+template <typename T>
+void launch_p_update(int n, T* p, const T* z, CgScalars* sc, hipStream_t st);
+void launch_scalar_alpha(CgScalars* sc, hipStream_t st);
+void launch_scalar_beta(CgScalars* sc, hipStream_t st);
+
+template <typename T>
+void launch_axpby(int n, T a, const T* x, T b, T* y, hipStream_t st);
+template <typename T>
+void launch_dot(int n, const T* a, const T* b, double* partial, hipStream_t st);
+void launch_cast_f64_f32(int n, const double* x, double scale, float* y, hipStream_t st);
+void launch_add_f32_f64(int n, const float* x, double scale, double* y, hipStream_t st);
+void launch_residual(const DevSpace& S, int n, const double* b, const double* Ax, double* r, double* partial, hipStream_t st);
+
+// RHS: bl[e][i] = 6V/6 * sum_q w_q f[e][q] phi_q[i] for elements [e0, e1)
+void launch_rhs_local(const DevSpace& S, int e0, int e1, int nq, const double* wphi /*nq x K, w folded*/,
+                      const double* fq, double* bl, hipStream_t st);
+void launch_qp_coords(const DevSpace& S, const double* xyz_elem /*nt x 12*/, int e0, int e1, int nq,
+                      const double* pts /*nq x 3*/, double* out, hipStream_t st);
+}  // namespace tsem
