@@ -85,21 +85,54 @@ def main():
             raise RuntimeError("solve did not reach rel-res 1e-10: %s" % st)
         return dt, st
 
+    import ctypes
+    from tsem_amd import lib as tl
+
     for _ in range(args.warmup):
         dt, st = solve(True)
         log("warmup mixed %.3f s %s" % (dt, st))
     times, stats = [], []
+    tl.check(tl.load().tsem_timing(1))
     for _ in range(args.steps):
         dt, st = solve(True)
         times.append(dt)
         stats.append(st)
         log("mixed %.3f s %s" % (dt, st))
+    kt = np.zeros(8)
+    tl.check(tl.load().tsem_timing_read(tl.dptr(kt)))
+    tl.check(tl.load().tsem_timing(0))
     t64 = []
     for _ in range(args.fp64_steps):
         dt, st64 = solve(False)
         t64.append(dt)
         log("fp64 %.3f s %s" % (dt, st64))
     mean = float(np.mean(times))
+    # roofline of the dominant kernel of the timed region (HIP events on the
+    # library stream): element kernel (MFMA, FP32) vs scatter (HBM)
+    K, E, N = sp.K, sp.nt, sp.ndof
+    Kd = args.p * (args.p + 1) * (args.p + 2) // 6
+    elem_ms = kt[4] / max(kt[5], 1)
+    scat_ms = kt[6] / max(kt[7], 1)
+    flops_elem = 12.0 * Kd * K * E                        # two GEMMs through the derivative factors
+    nnz_sc = float(sp.nmap)
+    bytes_scat = 4.0 * E * K + 4.0 * nnz_sc + 4.0 * (N + 1) + 4.0 * N + 4.0 * N + 1.0 * N  # yl, codes, ptr, y, x, mask
+    bytes_elem = 4.0 * N + 80.0 * E + 24.0 * E + 4.0 * E * K  # x (unique), descriptors, factors, yl
+    elem_tot, scat_tot = kt[4], kt[6]
+    kernels = {
+        "element_fp32": {"avg_ms": elem_ms, "launches": int(kt[5]), "tflops": flops_elem / (elem_ms * 1e-3) / 1e12 if kt[5] else None,
+                         "algorithmic_flop": flops_elem, "algorithmic_bytes": bytes_elem,
+                         "gbs": bytes_elem / (elem_ms * 1e-3) / 1e9 if kt[5] else None},
+        "scatter_fp32": {"avg_ms": scat_ms, "launches": int(kt[7]), "algorithmic_bytes": bytes_scat,
+                         "gbs": bytes_scat / (scat_ms * 1e-3) / 1e9 if kt[7] else None},
+    }
+    if elem_tot >= scat_tot:
+        ach = kernels["element_fp32"]["tflops"]
+        roof = {"kernel": "k_apply_mfma<float> (element operator, FP32 MFMA)", "bound": "mfma", "achieved": ach,
+                "peak": 157.3, "unit": "TFLOP/s", "frac": (ach / 157.3) if ach else None, "traffic": None}
+    else:
+        ach = kernels["scatter_fp32"]["gbs"]
+        roof = {"kernel": "k_scatter<float> (deterministic scatter)", "bound": "hbm", "achieved": ach, "peak": 8000.0,
+                "unit": "GB/s", "frac": (ach / 8000.0) if ach else None, "traffic": None}
     value = sp.ndof / mean
     fp64_value = sp.ndof / float(np.mean(t64)) if t64 else None
     out = {
@@ -125,7 +158,8 @@ def main():
         "outer_mixed": stats[-1]["outer"],
         "iterations_fp64": st64["iterations"] if t64 else None,
         "relres_mixed": stats[-1]["relres"],
-        "roofline": None,
+        "roofline": roof,
+        "kernels": kernels,
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -5,8 +5,9 @@
  * tsem_last_error() describes the cause.  Vectors marked "device" are HIP
  * device pointers of length n_dof (double or float as stated); all work runs
  * on the context's stream and the calls return after it completed unless
- * stated otherwise.  One context per host thread; calls on one context are
- * serialized.
+ * stated otherwise.  Device buffers passed in must be complete (no pending
+ * writes on other streams): the library works on its own stream.  One
+ * context per host thread; calls on one context are serialized.
  *
  * Each function names the TSEMLib interface it replaces (file:line in the
  * reference tree).
@@ -83,14 +84,23 @@ int tsem_space_sum_dof(const tsem_space* sp, int32_t* sum_dof);
  * pair weights w[nt][6] (2/(3V) x dihedral factor, src/Discretize.cpp:1355-1370). */
 int tsem_space_ref_matrices(const tsem_space* sp, double* mass, double* stiff6, double* pair_w);
 
+/* Factorizations the element kernels use (host): dfac[3][Kd][K] with
+ * dfac_a^T dfac_b = grad2[3a+b] = int d_a phi_i d_b phi_j, Kd = C(M+2,3);
+ * mfac[K][K] with mfac^T mfac = mass.  No counterpart (the reference
+ * assembles instead). */
+int tsem_space_factors(const tsem_space* sp, double* dfac, double* mfac, double* grad2);
+
 /* Boundary marks per vertex, edge, face: TSEM::build_flag_bm(mesh)
  * (src/Discretize.cpp:107-118). */
 int tsem_set_marks(tsem_space* sp, const int32_t* mark_v, const int32_t* mark_e, const int32_t* mark_f);
 
-/* Points where boundary data are sampled (host): pv[nv][3]; pe[ne][q][3]
- * along the first-visit edge direction; pf[nf][q*q][3] in the first-visit
- * face order (src/Discretize.cpp:1797-1830). */
-int tsem_bc_points(const tsem_space* sp, double* pv, double* pe, double* pf);
+/* Boundary data are given on the geometries that carry a mark, in geometry
+ * index order.  tsem_bc_count: numbers of vertices, edges, faces with
+ * mark == flag.  tsem_bc_points: where the data are sampled (host):
+ * pv[nv_f][3]; pe[ne_f][q][3] along the first-visit edge direction;
+ * pf[nf_f][q*q][3] in the first-visit face order (src/Discretize.cpp:1797-1830). */
+int tsem_bc_count(const tsem_space* sp, int flag, int64_t* counts3);
+int tsem_bc_points(const tsem_space* sp, int flag, double* pv, double* pe, double* pf);
 
 /* Load vector, device: b = sum_e G_e^T V_e B^T W f_e  (TSEM::calc_rhs with
  * values at quadrature points, src/Discretize.cpp:1396-1418).  Call
@@ -101,13 +111,13 @@ int tsem_rhs_begin(tsem_space* sp);
 int tsem_rhs_add(tsem_space* sp, int e0, int e1, const double* fq_dev);
 int tsem_rhs_end(tsem_space* sp, double* b_dev);
 
-/* Neumann load on faces with mark == flag; g at face quadrature points
- * (host, nf x q*q; rows of other faces ignored).  b device, updated.
+/* Neumann load on faces with mark == flag; g at their quadrature points
+ * (host, nf_f x q*q, tsem_bc_points order).  b device, updated.
  * TSEM::impose_boundary_condition(rhs, mesh, g, flag) (src/Discretize.cpp:2010-2063). */
 int tsem_neumann(tsem_space* sp, const double* g_face_qp, int flag, double* b_dev);
 
 /* Dirichlet imposition on geometries with mark == flag: modal interpolation of
- * the data (uv[nv], ue[ne][q], uf[nf][q*q] host, at tsem_bc_points), b
+ * the data (uv[nv_f], ue[ne_f][q], uf[nf_f][q*q] host, at tsem_bc_points), b
  * updated and the operator switched to the eliminated form (rows/columns of
  * the constrained DOFs replaced by the diagonal).  Optionally returns the
  * imposed modal values (bval host, n_dof, NaN on free DOFs).
@@ -129,6 +139,18 @@ int tsem_apply(tsem_space* sp, double stiff, double mass, int constrained, int p
 int tsem_gather(tsem_space* sp, const double* x_dev, double* xl_dev);
 /* Transposed gather, device: y = sum_e G_e^T yl_e (K1 scatter, fixed order). */
 int tsem_scatter(tsem_space* sp, const double* yl_dev, double* y_dev);
+
+/* Kernel timing with HIP events recorded on the launching stream around the
+ * full-order operator kernels (process wide).  tsem_timing(1) resets and
+ * starts, tsem_timing(0) stops; tsem_timing_read fills out[8] = {ms, count}
+ * for: element kernel FP64, scatter FP64, element kernel FP32, scatter FP32
+ * (synchronizes the device).  No counterpart. */
+int tsem_timing(int enable);
+int tsem_timing_read(double* out);
+
+/* Element-kernel selection for the operator (process wide): 0 = MFMA
+ * kernel where it applies (default), 1 = scalar kernel.  No counterpart. */
+int tsem_set_apply_path(int path);
 
 /* Assembled diagonal (device copy out, n_dof). */
 int tsem_diagonal(tsem_space* sp, double* diag_dev);

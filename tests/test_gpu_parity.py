@@ -57,19 +57,25 @@ def test_scatter_bit_exact(pair):
     assert np.array_equal(y, yp)
 
 
+@pytest.mark.parametrize("path", [0, 1], ids=["mfma", "scalar"])
 @pytest.mark.parametrize("coef", [(1.0, 0.0), (0.0, 1.0), (0.5, -2.0)])
-def test_apply_matches_assembled(pair, coef):
+def test_apply_matches_assembled(pair, coef, path):
     import torch
+    from tsem_amd import lib
     osp, psp = pair
-    _say("apply", psp.M, coef)
-    osp.assemble(*coef)
-    x = np.random.default_rng(3).standard_normal(osp.ndof)
-    y_ref = osp.vmult(x)
-    y64 = psp.apply(torch.from_numpy(x).cuda(), stiff=coef[0], mass=coef[1]).cpu().numpy()
-    scale = np.abs(y_ref).max()
-    assert np.abs(y64 - y_ref).max() <= 1e-13 * scale
-    y32 = psp.apply(torch.from_numpy(x).float().cuda(), stiff=coef[0], mass=coef[1]).double().cpu().numpy()
-    assert np.abs(y32 - y_ref).max() <= 2e-5 * scale
+    _say("apply", psp.M, coef, "path", path)
+    lib.check(lib.load().tsem_set_apply_path(path))
+    try:
+        osp.assemble(*coef)
+        x = np.random.default_rng(3).standard_normal(osp.ndof)
+        y_ref = osp.vmult(x)
+        y64 = psp.apply(torch.from_numpy(x).cuda(), stiff=coef[0], mass=coef[1]).cpu().numpy()
+        scale = np.abs(y_ref).max()
+        assert np.abs(y64 - y_ref).max() <= 1e-13 * scale
+        y32 = psp.apply(torch.from_numpy(x).float().cuda(), stiff=coef[0], mass=coef[1]).double().cpu().numpy()
+        assert np.abs(y32 - y_ref).max() <= 2e-5 * scale
+    finally:
+        lib.check(lib.load().tsem_set_apply_path(0))
 
 
 def test_apply_deterministic(pair):

@@ -23,7 +23,10 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
-            L.load().tsem_ctx_destroy(self.h)
+            try:
+                L.load().tsem_ctx_destroy(self.h)
+            except Exception:  # interpreter shutdown: the process releases the device
+                pass
             self.h = None
 
     def __del__(self):
@@ -32,6 +35,12 @@ class Context:
 
 def _ptr(t):
     return C.c_void_p(t.data_ptr())
+
+
+def _ready():
+    """Finish pending torch work before the library (own stream) reads it."""
+    import torch
+    torch.cuda.current_stream().synchronize()
 
 
 class Space:
@@ -60,7 +69,10 @@ class Space:
 
     def close(self):
         if getattr(self, "h", None):
-            L.load().tsem_space_destroy(self.h)
+            try:
+                L.load().tsem_space_destroy(self.h)
+            except Exception:  # interpreter shutdown
+                pass
             self.h = None
 
     def __del__(self):
@@ -103,6 +115,14 @@ class Space:
         L.check(L.load().tsem_space_ref_matrices(self.h, L.dptr(Mr), L.dptr(S), L.dptr(w)))
         return Mr, S, w
 
+    def factors(self):
+        """(dfac[3][Kd][K], mfac[K][K], grad2[3][3][K][K]) of the element kernels."""
+        K = self.K
+        Kd = self.M * (self.M + 1) * (self.M + 2) // 6
+        d = np.zeros((3, Kd, K)); m = np.zeros((K, K)); g = np.zeros((3, 3, K, K))
+        L.check(L.load().tsem_space_factors(self.h, L.dptr(d), L.dptr(m), L.dptr(g)))
+        return d, m, g
+
     def diagonal_host(self):
         d = np.zeros(self.ndof)
         L.check(L.load().tsem_space_diagonal_host(self.h, L.dptr(d)))
@@ -119,11 +139,21 @@ class Space:
         self.set_marks(mv, me, mf)
         return mv, me, mf
 
-    def bc_points(self):
+    def bc_count(self, flag):
+        n = np.zeros(3, np.int64)
+        L.check(L.load().tsem_bc_count(self.h, flag, L.lptr(n)))
+        return [int(v) for v in n]
+
+    def bc_points(self, flag):
+        """Sample points of boundary data on the geometries with mark flag."""
         q = self.qorder
-        pv = np.zeros((self.nv, 3)); pe = np.zeros((self.ne, q, 3)); pf = np.zeros((self.nf, q * q, 3))
-        L.check(L.load().tsem_bc_points(self.h, L.dptr(pv), L.dptr(pe), L.dptr(pf)))
+        nv, ne, nf = self.bc_count(flag)
+        pv = np.zeros((nv, 3)); pe = np.zeros((ne, q, 3)); pf = np.zeros((nf, q * q, 3))
+        L.check(L.load().tsem_bc_points(self.h, flag, L.dptr(pv), L.dptr(pe), L.dptr(pf)))
         return pv, pe, pf
+
+    def marked_faces(self, flag):
+        return np.nonzero(self.marks[2] == flag)[0]
 
     # -------------------------------------------------------------- device --
     def zeros(self, dtype=None):
@@ -144,18 +174,22 @@ class Space:
             pts = torch.empty((e1 - e0, self.nq3, 3), dtype=torch.float64, device=self.device)
             L.check(lib.tsem_qp_coords(self.h, e0, e1, _ptr(pts)))
             fq = f(pts).contiguous()
+            _ready()
             L.check(lib.tsem_rhs_add(self.h, e0, e1, _ptr(fq)))
             del pts, fq
+        _ready()
         L.check(lib.tsem_rhs_end(self.h, _ptr(b)))
         return b
 
     def neumann(self, g_face_qp, flag, b):
         g = np.ascontiguousarray(g_face_qp, dtype=np.float64)
+        _ready()
         L.check(L.load().tsem_neumann(self.h, L.dptr(g), flag, _ptr(b)))
 
     def dirichlet(self, uv, ue, uf, flag, b):
         uv, ue, uf = (np.ascontiguousarray(a, dtype=np.float64) for a in (uv, ue, uf))
         bval = np.zeros(self.ndof)
+        _ready()
         L.check(L.load().tsem_dirichlet(self.h, L.dptr(uv), L.dptr(ue), L.dptr(uf), flag, _ptr(b), L.dptr(bval)))
         return bval
 
@@ -166,22 +200,27 @@ class Space:
         import torch
         prec = 64 if x.dtype == torch.float64 else 32
         y = torch.empty_like(x) if out is None else out
+        _ready()
         L.check(L.load().tsem_apply(self.h, stiff, mass, 1 if constrained else 0, prec, _ptr(x), _ptr(y)))
         return y
 
     def gather(self, x):
         import torch
         xl = torch.empty((self.nt, self.K), dtype=torch.float64, device=self.device)
+        _ready()
         L.check(L.load().tsem_gather(self.h, _ptr(x), _ptr(xl)))
         return xl
 
     def scatter(self, yl):
         y = self.zeros()
-        L.check(L.load().tsem_scatter(self.h, _ptr(yl.contiguous()), _ptr(y)))
+        yl = yl.contiguous()
+        _ready()
+        L.check(L.load().tsem_scatter(self.h, _ptr(yl), _ptr(y)))
         return y
 
     def diagonal(self):
         d = self.zeros()
+        _ready()
         L.check(L.load().tsem_diagonal(self.h, _ptr(d)))
         return d
 
@@ -192,6 +231,7 @@ class Space:
         o = L.PcgOpts(tol, criterion, max_iter, precond, 1 if mixed else 0, inner_tol, inner_max, smooth_steps,
                       check_every)
         s = L.PcgStats()
+        _ready()
         rc = L.load().tsem_pcg(self.h, _ptr(b), _ptr(x), C.byref(o), C.byref(s))
         if rc != L.TSEM_OK and (rc != L.TSEM_ENOCONV or raise_on_noconv):
             L.check(rc)
@@ -207,11 +247,12 @@ def poisson_setup(sp, problem, dirichlet_planes=0b000001, neumann_planes=0b11111
     from .problems import cube_face_normal
     sp.set_cube_marks(dirichlet_planes, neumann_planes)
     b = sp.rhs(lambda p: problem.f(p))
-    pv, pe, pf = sp.bc_points()
     ev, fv, _, _ = sp.topology()
-    normals = cube_face_normal(sp.xyz[fv])
-    gq = problem.dudn(pf, [normals[:, None, d] for d in range(3)])
+    _, _, pf2 = sp.bc_points(2)
+    normals = cube_face_normal(sp.xyz[fv[sp.marked_faces(2)]])
+    gq = problem.dudn(pf2, [normals[:, None, d] for d in range(3)])
     sp.neumann(gq, 2, b)
+    pv, pe, pf = sp.bc_points(1)
     sp.bval = sp.dirichlet(problem.u(pv), problem.u(pe), problem.u(pf), 1, b)
     torch.cuda.synchronize()
     return b

@@ -7,6 +7,7 @@
 #include <dlfcn.h>
 
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -74,10 +75,19 @@ double jacobi(double a, double b, int n, double x) {
 
 }  // namespace
 
+// A context stays alive until its last space is destroyed, whatever order
+// the caller releases them in.
 struct tsem_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  int nspaces = 0;
+  bool closing = false;
 };
+
+static void ctx_release(tsem_ctx* c) {
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
 
 struct tsem_space {
   tsem_ctx* ctx = nullptr;
@@ -89,7 +99,7 @@ struct tsem_space {
   DevSpace d;
   // device storage
   DevBuf desc, slot, ft_ptr, ft_key, ft_w, ghat, ghat32, sixv, sixv32, dT, dN, dT32, dN32, mT, mN, mT32, mN32;
-  DevBuf sc_ptr, sc_code, sc_w, diag_d, invdiag, invdiag32, cmask_d, xe, pts3, wphi;
+  DevBuf sc_ptr, sc_code, sc_w, diag_d, invdiag, invdiag32, cmask_d, xe, pts3, wphi, Dp64, Dp32;
   DevBuf yl64, yl32, bl, partial, scal;
   // solver vectors
   DevBuf v64[6], v32[6];
@@ -155,8 +165,8 @@ int tsem_ctx_create(int device, tsem_ctx** out) {
 int tsem_ctx_destroy(tsem_ctx* ctx) {
   API_BEGIN
   if (!ctx) return TSEM_OK;
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-  delete ctx;
+  if (ctx->nspaces > 0) ctx->closing = true;
+  else ctx_release(ctx);
   API_END
 }
 
@@ -242,6 +252,15 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   for (int m = 0; m < K; ++m)
     for (int i = 0; i < K; ++i) mT[(size_t)i * K + m] = S.ref.mfac[(size_t)m * K + i];
   upload(sp->dT, dT, st); upload(sp->dN, dN, st); upload(sp->mT, mT, st); upload(sp->mN, mN, st);
+  // padded factors for the MFMA kernel: Dp[a Mp + m][i], Mp = roundup(Kd, 16), Cp = roundup(K, 16)
+  d.Mp = (Kd + 15) & ~15;
+  d.Cp = (K + 15) & ~15;
+  std::vector<double> Dp((size_t)3 * d.Mp * d.Cp, 0.0);
+  for (int a = 0; a < 3; ++a)
+    for (int m = 0; m < Kd; ++m)
+      for (int i = 0; i < K; ++i) Dp[(size_t)(a * d.Mp + m) * d.Cp + i] = S.ref.dfac[a][(size_t)m * K + i];
+  upload(sp->Dp64, Dp, st);
+  upload(sp->Dp32, std::vector<float>(Dp.begin(), Dp.end()), st);
   upload(sp->dT32, std::vector<float>(dT.begin(), dT.end()), st);
   upload(sp->dN32, std::vector<float>(dN.begin(), dN.end()), st);
   upload(sp->mT32, std::vector<float>(mT.begin(), mT.end()), st);
@@ -274,7 +293,9 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   d.sc_ptr = sp->sc_ptr.as<int32_t>(); d.sc_code = sp->sc_code.as<int32_t>(); d.sc_w = sp->sc_w.as<double>();
   d.diag = sp->diag_d.as<double>(); d.invdiag = sp->invdiag.as<double>(); d.invdiag32 = sp->invdiag32.as<float>();
   d.cmask = sp->cmask_d.as<uint8_t>();
+  d.Dp64 = sp->Dp64.as<double>(); d.Dp32 = sp->Dp32.as<float>();
   HIPCHK(hipStreamSynchronize(st));
+  ctx->nspaces++;
   *out = sp.release();
   API_END
 }
@@ -282,8 +303,10 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
 int tsem_space_destroy(tsem_space* sp) {
   API_BEGIN
   if (sp) {
-    if (sp->ctx) (void)hipStreamSynchronize(sp->ctx->stream);
+    tsem_ctx* c = sp->ctx;
+    if (c) (void)hipStreamSynchronize(c->stream);
     delete sp;
+    if (c && --c->nspaces == 0 && c->closing) ctx_release(c);
   }
   API_END
 }
@@ -363,6 +386,17 @@ int tsem_space_ref_matrices(const tsem_space* sp, double* mass, double* stiff6, 
   API_END
 }
 
+int tsem_space_factors(const tsem_space* sp, double* dfac, double* mfac, double* grad2) {
+  API_BEGIN
+  if (!sp) throw std::invalid_argument("null space");
+  const RefElement& R = sp->h.ref;
+  size_t n = (size_t)R.Kd * R.K;
+  if (dfac) for (int a = 0; a < 3; ++a) std::memcpy(dfac + a * n, R.dfac[a].data(), n * 8);
+  if (mfac) std::memcpy(mfac, R.mfac.data(), (size_t)R.K * R.K * 8);
+  if (grad2) for (int k = 0; k < 9; ++k) std::memcpy(grad2 + k * (size_t)R.K * R.K, R.grad2[k].data(), (size_t)R.K * R.K * 8);
+  API_END
+}
+
 int tsem_set_marks(tsem_space* sp, const int32_t* mv, const int32_t* me, const int32_t* mf) {
   API_BEGIN
   if (!sp || !mv || !me || !mf) throw std::invalid_argument("null argument");
@@ -373,29 +407,53 @@ int tsem_set_marks(tsem_space* sp, const int32_t* mv, const int32_t* me, const i
   API_END
 }
 
-int tsem_bc_points(const tsem_space* sp, double* pv, double* pe, double* pf) {
+// geometries of dimension dim carrying boundary mark `flag`, in index order
+static std::vector<int> marked(const Space& S, int dim, int flag) {
+  std::vector<int> out;
+  if (S.mark[dim].empty()) throw std::logic_error("boundary marks not set");
+  for (size_t g = 0; g < S.mark[dim].size(); ++g)
+    if (S.mark[dim][g] == flag) out.push_back((int)g);
+  return out;
+}
+
+int tsem_bc_count(const tsem_space* sp, int flag, int64_t* n3) {
+  API_BEGIN
+  if (!sp || !n3) throw std::invalid_argument("null argument");
+  for (int d = 0; d < 3; ++d) n3[d] = (int64_t)marked(sp->h, d, flag).size();
+  API_END
+}
+
+int tsem_bc_points(const tsem_space* sp, int flag, double* pv, double* pe, double* pf) {
   API_BEGIN
   if (!sp) throw std::invalid_argument("null space");
   const Space& S = sp->h;
   int q = sp->nq1, q2 = sp->nq2;
-  if (pv) std::memcpy(pv, S.xyz.data(), S.xyz.size() * 8);
-  if (pe)
-    for (int e = 0; e < S.topo.ne; ++e) {
-      const double* a = &S.xyz[3 * (size_t)S.edge_node[e][0]];
-      const double* b = &S.xyz[3 * (size_t)S.edge_node[e][1]];
+  if (pv) {
+    auto G = marked(S, 0, flag);
+    for (size_t j = 0; j < G.size(); ++j)
+      for (int c = 0; c < 3; ++c) pv[3 * j + c] = S.xyz[3 * (size_t)G[j] + c];
+  }
+  if (pe) {
+    auto G = marked(S, 1, flag);
+    for (size_t j = 0; j < G.size(); ++j) {
+      const double* a = &S.xyz[3 * (size_t)S.edge_node[G[j]][0]];
+      const double* b = &S.xyz[3 * (size_t)S.edge_node[G[j]][1]];
       for (int k = 0; k < q; ++k)
-        for (int c = 0; c < 3; ++c) pe[((size_t)e * q + k) * 3 + c] = a[c] * (1 - sp->t1[k]) + b[c] * sp->t1[k];
+        for (int c = 0; c < 3; ++c) pe[(j * q + k) * 3 + c] = a[c] * (1 - sp->t1[k]) + b[c] * sp->t1[k];
     }
-  if (pf)
-    for (int f = 0; f < S.topo.nf; ++f) {
-      const double* n0 = &S.xyz[3 * (size_t)S.face_node[f][0]];
-      const double* n1 = &S.xyz[3 * (size_t)S.face_node[f][1]];
-      const double* n2 = &S.xyz[3 * (size_t)S.face_node[f][2]];
+  }
+  if (pf) {
+    auto G = marked(S, 2, flag);
+    for (size_t j = 0; j < G.size(); ++j) {
+      const double* n0 = &S.xyz[3 * (size_t)S.face_node[G[j]][0]];
+      const double* n1 = &S.xyz[3 * (size_t)S.face_node[G[j]][1]];
+      const double* n2 = &S.xyz[3 * (size_t)S.face_node[G[j]][2]];
       for (int k = 0; k < q2; ++k) {
         const double* bq = &sp->p2[3 * (size_t)k];
-        for (int c = 0; c < 3; ++c) pf[((size_t)f * q2 + k) * 3 + c] = n0[c] * bq[2] + n1[c] * bq[0] + n2[c] * bq[1];
+        for (int c = 0; c < 3; ++c) pf[(j * q2 + k) * 3 + c] = n0[c] * bq[2] + n1[c] * bq[0] + n2[c] * bq[1];
       }
     }
+  }
   API_END
 }
 
@@ -476,8 +534,9 @@ int tsem_neumann(tsem_space* sp, const double* gq, int flag, double* b_dev) {
   std::vector<double> b(S.ndof);
   HIPCHK(hipMemcpyAsync(b.data(), b_dev, S.ndof * 8, hipMemcpyDeviceToHost, sp->st()));
   HIPCHK(hipStreamSynchronize(sp->st()));
-  for (int f = 0; f < S.topo.nf; ++f) {
-    if (S.mark[2][f] != flag) continue;
+  auto faces = marked(S, 2, flag);
+  for (size_t jf = 0; jf < faces.size(); ++jf) {
+    int f = faces[jf];
     const double* p0 = &S.xyz[3 * (size_t)S.face_node[f][0]];
     const double* p1 = &S.xyz[3 * (size_t)S.face_node[f][1]];
     const double* p2 = &S.xyz[3 * (size_t)S.face_node[f][2]];
@@ -489,7 +548,7 @@ int tsem_neumann(tsem_space* sp, const double* gq, int flag, double* b_dev) {
     std::vector<double> ce(3 * (size_t)nde, 0.0), cf(ndf, 0.0);
     for (int k = 0; k < q2; ++k) {
       const double* bq = &sp->p2[3 * (size_t)k];
-      double wg = sp->w2[k] * gq[(size_t)f * q2 + k] * area;
+      double wg = sp->w2[k] * gq[jf * q2 + k] * area;
       for (int j = 0; j < 3; ++j) cv[j] += wg * bq[(j + 2) % 3];
       double x = bq[0], y = bq[1], r = bq[2];
       for (int dd = 0; dd < nde; ++dd) {
@@ -555,19 +614,19 @@ int tsem_dirichlet(tsem_space* sp, const double* uv, const double* ue, const dou
   std::vector<uint8_t> newmask(S.ndof, 0);
   if (bval) for (int64_t i = 0; i < S.ndof; ++i) bval[i] = NAN;
   for (int dim = 0; dim <= 2; ++dim) {
-    int ng = dim == 0 ? S.topo.nv : dim == 1 ? S.topo.ne : S.topo.nf;
-    for (int g = 0; g < ng; ++g) {
-      if (S.mark[dim][g] != flag) continue;
+    auto geos = marked(S, dim, flag);
+    for (size_t jg = 0; jg < geos.size(); ++jg) {
+      int g = geos[jg];
       int64_t row0 = S.geo_dof(dim, g);
       std::vector<double> val(S.ndg[dim], 0.0);
-      if (dim == 0) val[0] = uv[g];
+      if (dim == 0) val[0] = uv[jg];
       if (dim == 1) {
         int64_t ls = S.geo_dof(0, S.edge_node[g][0]), le = S.geo_dof(0, S.edge_node[g][1]);
         double cs = b[ls] / dg[ls], ce = b[le] / dg[le];
         for (int l = 2; l <= M; ++l) {
           double c = 0;
           for (int k = 0; k < q1; ++k)
-            c += sp->w1[k] * (ue[(size_t)g * q1 + k] - cs * (1 - sp->t1[k]) - ce * sp->t1[k]) * ie1[(size_t)k * nde + l - 2];
+            c += sp->w1[k] * (ue[jg * q1 + k] - cs * (1 - sp->t1[k]) - ce * sp->t1[k]) * ie1[(size_t)k * nde + l - 2];
           val[l - 2] = c * -l * (2 * l - 1) / (2 * (l - 1));
         }
       }
@@ -590,7 +649,7 @@ int tsem_dirichlet(tsem_space* sp, const double* uv, const double* ue, const dou
             for (int k = 0; k < q2; ++k) {
               const double* bq = &sp->p2[3 * (size_t)k];
               double x = bq[0], y = bq[1], r = bq[2];
-              double cp = uf[(size_t)g * q2 + k];
+              double cp = uf[jg * q2 + k];
               for (int j = 0; j < 3; ++j) cp -= cvt[j] * bq[(j + 2) % 3];
               for (int l = 2; l <= M; ++l)
                 cp += 2 * (x * y * ved[l - 2] * eb[(size_t)k * nde + l - 2] + y * r * ved[nde + l - 2] * eb[(size_t)k * nde + l - 2] +
@@ -675,6 +734,26 @@ int tsem_scatter(tsem_space* sp, const double* yl, double* y) {
   API_END
 }
 
+int tsem_timing(int enable) {
+  API_BEGIN
+  timing_enable(enable != 0);
+  API_END
+}
+
+int tsem_timing_read(double* out) {
+  API_BEGIN
+  if (!out) throw std::invalid_argument("null argument");
+  timing_read(out);
+  API_END
+}
+
+int tsem_set_apply_path(int path) {
+  API_BEGIN
+  if (path < 0 || path > 1) throw std::invalid_argument("path must be 0 (auto) or 1 (scalar)");
+  set_apply_path(path);
+  API_END
+}
+
 int tsem_diagonal(tsem_space* sp, double* dd) {
   API_BEGIN
   if (!sp || !dd) throw std::invalid_argument("null argument");
@@ -737,6 +816,8 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
   };
   double res = resid(hs);
   int it = 0;
+  static const bool trace = getenv("TSEM_PCG_TRACE") != nullptr;
+  if (trace) fprintf(stderr, "[tsem pcg] start res %.3e bnorm %.3e rz %.3e\n", res, bnorm, hs.rz);
   while (res >= tol && it <= max_iter) {
     int nrun = std::max(1, check_every);
     for (int k = 0; k < nrun && it <= max_iter; ++k) {
@@ -751,6 +832,9 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
     HIPCHK(hipMemcpyAsync(&hs, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     res = resid(hs);
+    if (trace)
+      fprintf(stderr, "[tsem pcg%s] it %d res %.3e l1 %.3e l2 %.3e rz %.3e pq %.3e alpha %.3e beta %.3e\n",
+              sizeof(T) == 8 ? "64" : "32", it, res, hs.l1, std::sqrt(hs.l2), hs.rz, hs.pq, hs.alpha, hs.beta);
     if (!std::isfinite(res)) break;
   }
   *iters = it;

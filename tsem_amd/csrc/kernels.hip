@@ -11,6 +11,10 @@
 //  CG/vector: fused PCG updates with deterministic two-stage reductions.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <utility>
+#include <vector>
+
 #include "kernels.hpp"
 
 namespace tsem {
@@ -28,6 +32,7 @@ __device__ inline T ldx(const T* __restrict__ x, const uint8_t* __restrict__ cma
 // result is bit-identical to a plain sequential evaluation
 template <typename T>
 __device__ inline T gather_slot(const DevSpace& S, int e, int i, const T* __restrict__ x, int constrained) {
+#pragma clang fp contract(off)
   const ElemDesc& D = S.desc[e];
   int code = S.slot[i];
   int kind = code & 3, geo = (code >> 2) & 7, sub = code >> 5;
@@ -56,8 +61,7 @@ __device__ inline T gather_slot(const DevSpace& S, int e, int i, const T* __rest
       g = D.f[geo] + idx;
     }
     T xv = ldx(x, S.cmask, g, constrained);
-    if constexpr (sizeof(T) == 8) acc = __dadd_rn(acc, __dmul_rn(xv, w));
-    else acc += xv * w;
+    acc += xv * w;
   }
   return acc;
 }
@@ -190,6 +194,7 @@ template <typename T>
 __global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict__ yl, T* __restrict__ y,
                                                  const T* __restrict__ xin, int constrained,
                                                  double* __restrict__ partial) {
+#pragma clang fp contract(off)
   __shared__ double red[BLK];
   int g = blockIdx.x * BLK + threadIdx.x;
   double d = 0.0;
@@ -201,14 +206,9 @@ __global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict
       for (int t = S.sc_ptr[g]; t < S.sc_ptr[g + 1]; ++t) {
         int code = S.sc_code[t];
         T v = yl[code & 0x3fffffff];
-        if (code < 0) {
-          if constexpr (sizeof(T) == 8) v = __dmul_rn(v, S.sc_w[t]);
-          else v = v * (T)S.sc_w[t];
-        } else if (code & 0x40000000) {
-          v = -v;
-        }
-        if constexpr (sizeof(T) == 8) acc = __dadd_rn(acc, v);
-        else acc += v;
+        if (code < 0) v = v * (T)S.sc_w[t];
+        else if (code & 0x40000000) v = -v;
+        acc += v;
       }
     }
     y[g] = acc;
@@ -224,14 +224,76 @@ __global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// ---- optional per-kernel timing with HIP events on the launching stream ----
+namespace {
+struct KTimer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<std::pair<int, size_t>> rec;  // (kind, index of start event)
+};
+KTimer g_kt;
+}  // namespace
+
+void timing_enable(bool on) {
+  if (on && g_kt.pool.empty()) {
+    g_kt.pool.resize(2 * 40000);
+    for (auto& e : g_kt.pool) (void)hipEventCreate(&e);
+  }
+  g_kt.on = on;
+  g_kt.used = 0;
+  g_kt.rec.clear();
+}
+
+// kinds: 0 element kernel FP64, 1 scatter FP64, 2 element FP32, 3 scatter FP32
+void timing_read(double* out) {
+  for (int k = 0; k < 8; ++k) out[k] = 0;
+  (void)hipDeviceSynchronize();
+  for (auto& r : g_kt.rec) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, g_kt.pool[r.second], g_kt.pool[r.second + 1]) == hipSuccess) {
+      out[2 * r.first] += ms;
+      out[2 * r.first + 1] += 1;
+    }
+  }
+}
+
+static inline int timing_begin(int kind, hipStream_t st) {
+  if (!g_kt.on || g_kt.used + 2 > g_kt.pool.size()) return -1;
+  size_t i = g_kt.used;
+  g_kt.used += 2;
+  (void)hipEventRecord(g_kt.pool[i], st);
+  g_kt.rec.push_back({kind, i});
+  return (int)i;
+}
+static inline void timing_end(int h, hipStream_t st) {
+  if (h >= 0) (void)hipEventRecord(g_kt.pool[h + 1], st);
+}
+
+static int apply_path_from_env() {
+  const char* v = getenv("TSEM_APPLY_PATH");
+  return (v && v[0] == '1') ? 1 : 0;
+}
+static int g_apply_path = apply_path_from_env();
+void set_apply_path(int path) { g_apply_path = path; }
+
 template <typename T>
 static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, double* partial, hipStream_t st) {
-  constexpr int NE = 8;
-  size_t lds = (size_t)NE * (2 * S.K + 3 * S.Kd) * sizeof(T);
-  int nb = (S.nt + NE - 1) / NE;
-  hipLaunchKernelGGL((k_apply<T, NE>), dim3(nb), dim3(BLK), lds, st, S, op, x, yl);
+  const int kind = (sizeof(T) == 8) ? 0 : 2;
+  int th = (op.Kl == 0) ? timing_begin(kind, st) : -1;
+  if (op.mass == 0.0 && g_apply_path == 0 && mfma_supported(S)) {
+    launch_apply_mfma<T>(S, op, x, yl, st);
+  } else {
+    constexpr int NE = 8;
+    size_t lds = (size_t)NE * (2 * S.K + 3 * S.Kd) * sizeof(T);
+    int nb = (S.nt + NE - 1) / NE;
+    hipLaunchKernelGGL((k_apply<T, NE>), dim3(nb), dim3(BLK), lds, st, S, op, x, yl);
+  }
+  timing_end(th, st);
   int ng = (S.ndof + BLK - 1) / BLK;
+  th = (op.Kl == 0) ? timing_begin(kind + 1, st) : -1;
   hipLaunchKernelGGL(k_scatter<T>, dim3(ng), dim3(BLK), 0, st, S, yl, y, x, op.constrained, partial);
+  timing_end(th, st);
 }
 
 template <>

@@ -35,6 +35,10 @@ struct DevSpace {
   const double* invdiag = nullptr;
   const float* invdiag32 = nullptr;
   const uint8_t* cmask = nullptr;    // 1 where the DOF is eliminated (Dirichlet)
+  // padded derivative factors for the MFMA path: Dp[3 Mp][Cp]
+  int Mp = 0, Cp = 0;
+  const double* Dp64 = nullptr;
+  const float* Dp32 = nullptr;
 };
 
 // operator selection
@@ -53,6 +57,13 @@ struct CgScalars {
 void launch_gather(const DevSpace& S, const double* x, double* xl, int constrained, hipStream_t st);
 template <typename T>
 void launch_apply(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, double* partial, hipStream_t st);
+template <typename T>
+void launch_apply_mfma(const DevSpace& S, const OpCoef& op, const T* x, T* yl, hipStream_t st);
+bool mfma_supported(const DevSpace& S);
+void timing_enable(bool on);
+void timing_read(double* out);
+// 0: auto (MFMA when supported), 1: force the scalar element kernel
+void set_apply_path(int path);
 template <typename T>
 void launch_scatter(const DevSpace& S, const T* yl, T* y, hipStream_t st);
 

@@ -31,7 +31,9 @@ __global__ void k_cheb(int n, const T* __restrict__ b, const T* __restrict__ Ax,
   int i = blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
   T r = b[i] - Ax[i];
-  T dn = c1 * dv[i] + c2 * inv[i] * r;
+  // first step (c1 = 0) must not read dv: it may hold stale data of the other precision
+  T dn = c2 * inv[i] * r;
+  if (c1 != T(0)) dn += c1 * dv[i];
   dv[i] = dn;
   x[i] += dn;
 }

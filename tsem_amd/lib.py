@@ -44,7 +44,9 @@ SIGNATURES = {
     "tsem_space_sum_dof": [_vp, _ip],
     "tsem_space_ref_matrices": [_vp, _dp, _dp, _dp],
     "tsem_set_marks": [_vp, _ip, _ip, _ip],
-    "tsem_bc_points": [_vp, _dp, _dp, _dp],
+    "tsem_space_factors": [_vp, _dp, _dp, _dp],
+    "tsem_bc_count": [_vp, C.c_int, _lp],
+    "tsem_bc_points": [_vp, C.c_int, _dp, _dp, _dp],
     "tsem_qp_coords": [_vp, C.c_int, C.c_int, _vp],
     "tsem_rhs_begin": [_vp],
     "tsem_rhs_add": [_vp, C.c_int, C.c_int, _vp],
@@ -56,6 +58,9 @@ SIGNATURES = {
     "tsem_gather": [_vp, _vp, _vp],
     "tsem_scatter": [_vp, _vp, _vp],
     "tsem_diagonal": [_vp, _vp],
+    "tsem_set_apply_path": [C.c_int],
+    "tsem_timing": [C.c_int],
+    "tsem_timing_read": [_dp],
     "tsem_pcg": [_vp, _vp, _vp, C.POINTER(PcgOpts), C.POINTER(PcgStats)],
 }
 
