@@ -1,15 +1,20 @@
-// Element operator on the matrix cores (gfx950 MFMA).
+// Element operator on the matrix cores (gfx950 MFMA), in two kernels.
 //
-// For a tile of NE elements the stiffness action is two dense GEMMs through
-// the derivative factors D_a (orthonormal coefficients of d_a phi_i):
-//   U (3 Mp x NE) = D (3 Mp x K) X (K x NE)            X = gathered x_e
-//   V_a = sum_b Ghat_ab(e) U_b                          per column (element)
+// k_gather_tiles: element-local coefficients of 32-element tiles, written
+//   k-major inside the tile (xt[tile][k][32]) so the GEMM kernel streams them
+//   with coalesced loads.
+// k_elem_gemm: for a block of NE = 16 CT elements
+//   U (3 Mp x NE) = D (3 Mp x K) X (K x NE)            D_a: derivative factors
+//   V_a = sum_b Ghat_ab(e) U_b                          per element column
 //   Y (K x NE)   = D^T (K x 3 Mp) V (3 Mp x NE)
-// with v_mfma_{f32,f64}_16x16x4.  Rows of each a-block are padded to Mp (a
-// multiple of 16) so the three derivative components of one (mode, element)
-// land in the same lane and the 3x3 mixing happens in registers.  X and V
-// live in LDS; D is read from global memory (a few tens of KB, L1/L2
-// resident) in the padded layout Dp[3 Mp][Cp], Cp = roundup(K, 16).
+//   with v_mfma_{f32,f64}_16x16x4.  Each a-block of rows is padded to Mp (a
+//   multiple of 16) so the three derivative components of one (mode,
+//   element) sit in the same lane and the 3x3 mixing happens in registers;
+//   V goes through LDS between the GEMMs.  D is block triangular in degree
+//   (d_a phi_i has degree deg(phi_i) - 1): per tile the k ranges skip the
+//   structurally zero blocks (about 30% of the dense work at p = 6).
+//   Y leaves through an LDS transpose, element-major (yl[e][i]), so the
+//   scatter reads each element's modes from contiguous memory.
 //
 // MFMA 16x16x4 operand layout (one 64-lane wave): lane l supplies
 // A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator value j of lane l is
@@ -24,6 +29,7 @@ namespace tsem {
 namespace {
 constexpr int WAVES = 4;
 constexpr int NT = 64 * WAVES;
+constexpr int RMAX = 12;  // row tiles of GEMM 2 per block (K <= 192)
 
 template <typename T> struct Mma;
 template <> struct Mma<float> {
@@ -77,40 +83,91 @@ __device__ inline T gather_one(const DevSpace& S, int e, int i, const T* __restr
 
 }  // namespace
 
-// CT: number of 16-wide column tiles (NE = 16 CT elements per block)
+// one block per tile of TILE elements: the tile's element descriptors are
+// staged in LDS once, then each thread produces slots (k, element); writes
+// zeros for k >= Kl and for padding elements.  Dirichlet-eliminated DOFs
+// read as zero through the per-element geometry bits (vertex k: bit k, edge
+// k: bit 4+k, face k: bit 10+k, interior: bit 14).
+template <typename T>
+__global__ void __launch_bounds__(256) k_gather_tiles(DevSpace S, int Kl, const T* __restrict__ x, T* __restrict__ xt,
+                                                      int constrained) {
+  __shared__ ElemDesc sd[TILE];
+  __shared__ int sm[TILE];
+  const int tile = blockIdx.x;
+  const int e0 = tile * TILE;
+  {
+    const int32_t* src = reinterpret_cast<const int32_t*>(S.desc + e0);
+    int32_t* dst = reinterpret_cast<int32_t*>(sd);
+    constexpr int W = sizeof(ElemDesc) / 4;
+    int nvalid = min(TILE, S.nt - e0);
+    for (int s = threadIdx.x; s < TILE * W; s += 256) dst[s] = (s / W < nvalid) ? src[s] : 0;
+    for (int s = threadIdx.x; s < TILE; s += 256) sm[s] = (constrained && s < nvalid) ? S.gmask[e0 + s] : 0;
+  }
+  __syncthreads();
+  const int Kp = S.Kp;
+  T* out = xt + (size_t)tile * Kp * TILE;
+  for (int r = threadIdx.x; r < Kp * TILE; r += 256) {
+    int k = r / TILE, le = r % TILE, e = e0 + le;
+    T v = T(0);
+    if (e < S.nt && k < Kl) {
+      const ElemDesc& D = sd[le];
+      const int msk = sm[le];
+      int code = S.slot[k];
+      int kind = code & 3, geo = (code >> 2) & 7, sub = code >> 5;
+      if (kind == 0) {
+        v = ((msk >> geo) & 1) ? T(0) : x[D.v[geo]];
+      } else if (kind == 1) {
+        int ed = D.e[geo];
+        v = ((msk >> (4 + geo)) & 1) ? T(0) : x[(ed & 0x7fffffff) + sub];
+        if (ed < 0 && (sub & 1)) v = -v;
+      } else if (kind == 3) {
+        v = ((msk >> 14) & 1) ? T(0) : x[D.t + sub];
+      } else {
+        int fo = D.fo[geo];
+        int row = (fo & 7) * S.nface + sub;
+        const bool fmask = (msk >> (10 + geo)) & 1;
+        T acc = T(0);
+        for (int t = S.ft_ptr[row]; t < S.ft_ptr[row + 1]; ++t) {
+          int key = S.ft_key[t];
+          int ie = key & 3, idx = key >> 2;
+          T w = (T)S.ft_w[t];
+          T xv;
+          if (ie < 3) {
+            int le2 = (fo >> (3 + 3 * ie)) & 7;
+            if (((fo >> (12 + ie)) & 1) && (idx & 1)) w = -w;
+            xv = ((msk >> (4 + le2)) & 1) ? T(0) : x[(D.e[le2] & 0x7fffffff) + idx];
+          } else {
+            xv = fmask ? T(0) : x[D.f[geo] + idx];
+          }
+          acc += xv * w;
+        }
+        v = acc;
+      }
+    }
+    out[r] = v;
+  }
+}
+
 template <typename T, int CT>
-__global__ void __launch_bounds__(NT) k_apply_mfma(DevSpace S, OpCoef op, const T* __restrict__ Dp,
-                                                   const T* __restrict__ x, T* __restrict__ yl) {
+__global__ void __launch_bounds__(NT) k_elem_gemm(DevSpace S, OpCoef op, const T* __restrict__ Dp,
+                                                  const T* __restrict__ xt, T* __restrict__ yl) {
   using v4 = typename Mma<T>::v4;
   constexpr int NE = 16 * CT;
   extern __shared__ unsigned char smem_raw[];
+  T* V = reinterpret_cast<T*>(smem_raw);  // [3 Mp][NE]
   const int K = S.K;
   const int Kl = op.Kl > 0 ? op.Kl : K;
   const int Kdl = op.Kdl > 0 ? op.Kdl : S.Kd;
-  const int Mp = S.Mp, Cp = S.Cp;                 // padded a-block rows, padded columns of Dp
-  const int Kp = (Kl + 3) & ~3;                   // k extent of GEMM 1
-  const int Mt = (Kdl + 15) >> 4;                 // m tiles per a-block in use
-  const int Rt = (Kl + 15) >> 4;                  // output row tiles of GEMM 2
-  T* X = reinterpret_cast<T*>(smem_raw);          // [Kp][NE]
-  T* V = X + (size_t)((K + 3) & ~3) * NE;         // [3 Mp][NE]
-  T* G = V + (size_t)3 * Mp * NE;                 // [NE][6]
+  const int Mp = S.Mp, Cp = S.Cp, Kp = S.Kp;
+  const int Mt = (Kdl + 15) >> 4;
+  const int Rt = (Kl + 15) >> 4;
+  const int Kpl = (Kl + 3) & ~3;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
   const int e0 = blockIdx.x * NE;
   const T cst = (T)op.stiff;
 
-  // gather the tile: X[k][le]
-  for (int s = tid; s < Kp * NE; s += NT) {
-    int le = s % NE, k = s / NE, e = e0 + le;
-    X[s] = (e < S.nt && k < Kl) ? gather_one<T>(S, e, k, x, op.constrained) : T(0);
-  }
-  for (int s = tid; s < NE * 6; s += NT) {
-    int le = s / 6, e = e0 + le;
-    G[s] = (e < S.nt) ? cst * (sizeof(T) == 8 ? (T)S.ghat[6 * e + s % 6] : (T)S.ghat32[6 * e + s % 6]) : T(0);
-  }
-  __syncthreads();
-
-  const int lr = lane & 15, lk = lane >> 4;
-  // GEMM 1 + mixing: wave w owns m tiles w, w + WAVES, ...
+  // ---- GEMM 1 + mixing: wave w owns m tiles w, w + WAVES, ...
   for (int t = wave; t < Mt; t += WAVES) {
     v4 acc[3][CT];
 #pragma unroll
@@ -120,11 +177,19 @@ __global__ void __launch_bounds__(NT) k_apply_mfma(DevSpace S, OpCoef op, const 
     const T* d0 = Dp + (size_t)(0 * Mp + 16 * t + lr) * Cp + lk;
     const T* d1 = Dp + (size_t)(1 * Mp + 16 * t + lr) * Cp + lk;
     const T* d2 = Dp + (size_t)(2 * Mp + 16 * t + lr) * Cp + lk;
-    for (int k = 0; k < Kp; k += 4) {
+    const T* xb[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      int le = 16 * c + lr, e = e0 + le;
+      xb[c] = xt + (size_t)(e >> 5) * Kp * TILE + (size_t)lk * TILE + (e & 31);
+    }
+    const int kb = S.kbeg1[t];
+#pragma unroll 2
+    for (int k = kb; k < Kpl; k += 4) {
       T a0 = d0[k], a1 = d1[k], a2 = d2[k];
 #pragma unroll
       for (int c = 0; c < CT; ++c) {
-        T b = X[(k + lk) * NE + 16 * c + lr];
+        T b = xb[c][(size_t)k * TILE];
         acc[0][c] = Mma<T>::op(a0, b, acc[0][c]);
         acc[1][c] = Mma<T>::op(a1, b, acc[1][c]);
         acc[2][c] = Mma<T>::op(a2, b, acc[2][c]);
@@ -132,92 +197,104 @@ __global__ void __launch_bounds__(NT) k_apply_mfma(DevSpace S, OpCoef op, const 
     }
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
-      const T* g = G + (16 * c + lr) * 6;
-      T g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3], g4 = g[4], g5 = g[5];
+      int col = 16 * c + lr, e = e0 + col;
+      T g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0, g5 = 0;
+      if (e < S.nt) {
+        const T* g = (sizeof(T) == 8) ? (const T*)(S.ghat + 6 * (size_t)e) : (const T*)(S.ghat32 + 6 * (size_t)e);
+        g0 = cst * g[0]; g1 = cst * g[1]; g2 = cst * g[2]; g3 = cst * g[3]; g4 = cst * g[4]; g5 = cst * g[5];
+      }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         int m = 16 * t + Mma<T>::row(lk, j);
         T u0 = acc[0][c][j], u1 = acc[1][c][j], u2 = acc[2][c][j];
         T keep = (m < Kdl) ? T(1) : T(0);
-        int col = 16 * c + lr;
         V[(size_t)(0 * Mp + m) * NE + col] = keep * (g0 * u0 + g3 * u1 + g4 * u2);
         V[(size_t)(1 * Mp + m) * NE + col] = keep * (g3 * u0 + g1 * u1 + g5 * u2);
         V[(size_t)(2 * Mp + m) * NE + col] = keep * (g4 * u0 + g5 * u1 + g2 * u2);
       }
     }
   }
-  // m tiles beyond Mt are never written: clear them so GEMM 2 reads zeros
-  for (int s = tid; s < 3 * (Mp - 16 * Mt) * NE; s += NT) {
-    int per = (Mp - 16 * Mt) * NE;
-    int a = s / per, r = s % per;
-    V[(size_t)(a * Mp + 16 * Mt) * NE + r] = T(0);
-  }
   __syncthreads();
 
-  // GEMM 2: Y[i][e] = sum_r Dp[r][i] V[r][e]; wave w owns row tiles w, w + WAVES, ...
-  T* Y = X;  // reuse of the (dead) X and V regions: [16 Rt][NE]
-  v4 out[3][CT];
-  int nown = 0;
-  for (int t2 = wave; t2 < Rt; t2 += WAVES) {
-    v4 acc[CT];
+  // ---- GEMM 2: Y[i][e] = sum_r Dp[r][i] V[r][e].  Wave w owns column tile
+  // w % CT and row tiles w / CT + j (4 / CT), j = 0.. (compile-time slots)
+  constexpr int RSTEP = WAVES / CT;
+  constexpr int RW = (RMAX + RSTEP - 1) / RSTEP;  // row-tile slots per wave
+  const int c = wave % CT, r0 = wave / CT;
+  v4 acc[RW];
 #pragma unroll
-    for (int c = 0; c < CT; ++c) acc[c] = v4{0, 0, 0, 0};
-    for (int a = 0; a < 3; ++a) {
-      const T* dbase = Dp + (size_t)(a * Mp) * Cp + 16 * t2 + lr;
-      const T* vbase = V + (size_t)(a * Mp) * NE;
-      for (int k = 0; k < 16 * Mt; k += 4) {
-        T av = dbase[(size_t)(k + lk) * Cp];
+  for (int j = 0; j < RW; ++j) acc[j] = v4{0, 0, 0, 0};
+  int kend[RW];
 #pragma unroll
-        for (int c = 0; c < CT; ++c) acc[c] = Mma<T>::op(av, vbase[(k + lk) * NE + 16 * c + lr], acc[c]);
+  for (int j = 0; j < RW; ++j) {
+    int t2 = r0 + RSTEP * j;
+    kend[j] = (t2 < Rt) ? min(S.kend2[t2], 16 * Mt) : 0;
+  }
+  int kmax = 0;
+#pragma unroll
+  for (int j = 0; j < RW; ++j) kmax = max(kmax, kend[j]);
+  for (int a = 0; a < 3; ++a) {
+    const T* vb = V + (size_t)(a * Mp) * NE + 16 * c + lr;
+    const T* db = Dp + (size_t)(a * Mp + lk) * Cp + 16 * r0 + lr;
+    for (int k = 0; k < kmax; k += 4) {
+      T b = vb[(size_t)(k + lk) * NE];
+#pragma unroll
+      for (int j = 0; j < RW; ++j) {
+        if (k < kend[j]) {
+          T av = db[(size_t)k * Cp + 16 * RSTEP * j];
+          acc[j] = Mma<T>::op(av, b, acc[j]);
+        }
       }
     }
-    if (nown < 3) {
-#pragma unroll
-      for (int c = 0; c < CT; ++c) out[nown][c] = acc[c];
-    }
-    ++nown;
   }
-  __syncthreads();  // all waves done reading X (gather) before it is reused for Y
-  nown = 0;
-  for (int t2 = wave; t2 < Rt; t2 += WAVES) {
+  // ---- write Y element-major (yl[e][i]) through an LDS transpose
+  __syncthreads();  // every wave is done reading V
+  T* Y = V;         // [K][NE]
 #pragma unroll
-    for (int c = 0; c < CT; ++c)
+  for (int j = 0; j < RW; ++j) {
+    int t2 = r0 + RSTEP * j;
+    if (t2 < Rt) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) Y[(size_t)(16 * t2 + Mma<T>::row(lk, j)) * NE + 16 * c + lr] = out[nown][c][j];
-    ++nown;
+      for (int q = 0; q < 4; ++q) {
+        int i = 16 * t2 + Mma<T>::row(lk, q);
+        if (i < K) Y[(size_t)i * NE + 16 * c + lr] = acc[j][q];
+      }
+    }
   }
   __syncthreads();
-  // coalesced write yl[e][i] (zeros for modes beyond the level)
-  for (int s = tid; s < NE * K; s += NT) {
-    int le = s / K, i = s % K, e = e0 + le;
-    if (e < S.nt) yl[(size_t)e * K + i] = (i < Kl) ? Y[(size_t)i * NE + le] : T(0);
+  const int nval = min(NE, S.nt - e0);
+  for (int s = tid; s < nval * K; s += NT) {
+    int le = s / K, i = s % K;
+    yl[(size_t)(e0 + le) * K + i] = (i < Kl && i < 16 * Rt) ? Y[(size_t)i * NE + le] : T(0);
   }
 }
 
 template <typename T, int CT>
-static size_t lds_bytes(const DevSpace& S) {
-  constexpr int NE = 16 * CT;
-  return ((size_t)((S.K + 3) & ~3) * NE + (size_t)3 * S.Mp * NE + (size_t)NE * 6) * sizeof(T);
+static size_t gemm_lds(const DevSpace& S) {
+  size_t rows = (size_t)3 * S.Mp > (size_t)S.K ? (size_t)3 * S.Mp : (size_t)S.K;
+  return rows * 16 * CT * sizeof(T);
 }
 
 bool mfma_supported(const DevSpace& S) {
-  // GEMM 2 keeps at most 3 row tiles per wave in registers: K <= 16*3*WAVES;
-  // LDS must hold X, V and the tile's geometric factors
-  return S.Dp64 != nullptr && S.K <= 16 * 3 * WAVES && lds_bytes<double, 2>(S) <= 160 * 1024 &&
-         lds_bytes<float, 4>(S) <= 160 * 1024;
+  return S.Dp64 != nullptr && S.xt64 != nullptr && S.K <= 16 * RMAX && gemm_lds<float, 4>(S) <= 160 * 1024 &&
+         gemm_lds<double, 2>(S) <= 160 * 1024;
+}
+
+template <typename T, int CT>
+static void run(const DevSpace& S, const OpCoef& op, const T* Dp, T* xt, const T* x, T* yl, hipStream_t st) {
+  const int Kl = op.Kl > 0 ? op.Kl : S.K;
+  hipLaunchKernelGGL(k_gather_tiles<T>, dim3((unsigned)S.ntile), dim3(256), 0, st, S, Kl, x, xt, op.constrained);
+  int nb = (S.nt + 16 * CT - 1) / (16 * CT);
+  hipLaunchKernelGGL((k_elem_gemm<T, CT>), dim3(nb), dim3(NT), (gemm_lds<T, CT>(S)), st, S, op, Dp, xt, yl);
 }
 
 template <>
 void launch_apply_mfma<double>(const DevSpace& S, const OpCoef& op, const double* x, double* yl, hipStream_t st) {
-  constexpr int CT = 2;
-  int nb = (S.nt + 16 * CT - 1) / (16 * CT);
-  hipLaunchKernelGGL((k_apply_mfma<double, CT>), dim3(nb), dim3(NT), (lds_bytes<double, CT>(S)), st, S, op, S.Dp64, x, yl);
+  run<double, 2>(S, op, S.Dp64, S.xt64, x, yl, st);
 }
 template <>
 void launch_apply_mfma<float>(const DevSpace& S, const OpCoef& op, const float* x, float* yl, hipStream_t st) {
-  constexpr int CT = 4;
-  int nb = (S.nt + 16 * CT - 1) / (16 * CT);
-  hipLaunchKernelGGL((k_apply_mfma<float, CT>), dim3(nb), dim3(NT), (lds_bytes<float, CT>(S)), st, S, op, S.Dp32, x, yl);
+  run<float, 4>(S, op, S.Dp32, S.xt32, x, yl, st);
 }
 
 }  // namespace tsem

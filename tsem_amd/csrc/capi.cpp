@@ -100,7 +100,7 @@ struct tsem_space {
   // device storage
   DevBuf desc, slot, ft_ptr, ft_key, ft_w, ghat, ghat32, sixv, sixv32, dT, dN, dT32, dN32, mT, mN, mT32, mN32;
   DevBuf sc_ptr, sc_code, sc_w, diag_d, invdiag, invdiag32, cmask_d, xe, pts3, wphi, Dp64, Dp32;
-  DevBuf yl64, yl32, bl, partial, scal;
+  DevBuf yl64, yl32, bl, partial, scal, xt64, xt32, gmask;
   // solver vectors
   DevBuf v64[6], v32[6];
   std::unique_ptr<Pmg> pmg;
@@ -144,6 +144,36 @@ static std::string face_table_path() {
     return dir + "/data/face_table.txt";
   }
   return "data/face_table.txt";
+}
+
+// Upload the per-DOF elimination mask and derive the per-element geometry
+// bits the tiled gather reads (vertex k: bit k, edge k: bit 4+k, face k: bit
+// 10+k, interior: bit 14).  Eliminations act on whole geometries, so the
+// first DOF of a geometry speaks for all of them; checked here.
+static void upload_mask(tsem_space* sp, hipStream_t st) {
+  const Space& S = sp->h;
+  const auto& m = sp->cmask;
+  auto geo = [&](int64_t start, int n) -> int {
+    if (n <= 0) return 0;
+    int v = m[start];
+    for (int i = 1; i < n; ++i)
+      if (m[start + i] != v) throw std::logic_error("elimination mask is not uniform over a geometry");
+    return v;
+  };
+  std::vector<int32_t> gm(S.desc.size());
+  for (size_t e = 0; e < S.desc.size(); ++e) {
+    const ElemDesc& D = S.desc[e];
+    int b = 0;
+    for (int k = 0; k < 4; ++k) b |= geo(D.v[k], S.ndg[0] > 0 ? 1 : 0) << k;
+    for (int k = 0; k < 6; ++k) b |= geo(D.e[k] & 0x7fffffff, S.ndg[1]) << (4 + k);
+    for (int k = 0; k < 4; ++k) b |= geo(D.f[k], S.ndg[2]) << (10 + k);
+    b |= geo(D.t, S.ndg[3]) << 14;
+    gm[e] = b;
+  }
+  upload(sp->cmask_d, sp->cmask, st);
+  upload(sp->gmask, gm, st);
+  sp->d.cmask = sp->cmask_d.as<uint8_t>();
+  sp->d.gmask = sp->gmask.as<int32_t>();
 }
 
 extern "C" {
@@ -259,6 +289,31 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   for (int a = 0; a < 3; ++a)
     for (int m = 0; m < Kd; ++m)
       for (int i = 0; i < K; ++i) Dp[(size_t)(a * d.Mp + m) * d.Cp + i] = S.ref.dfac[a][(size_t)m * K + i];
+  // structural zeros: d_a phi_i has degree deg(i) - 1, and psi_m is
+  // orthogonal to P_d for m >= C(d+3, 3); so Dp[m][i] = 0 once
+  // m >= C(max(1, deg(i) - 1) + 3, 3).  Per 16-row tile: first column of
+  // GEMM 1 and row extent of GEMM 2.
+  {
+    auto degA = [&](int i) { auto a = S.ref.idx.mi[i]; return std::max(1, a[0] + a[1] + a[2]); };
+    auto lim = [&](int i) { return ModeIndex::count_upto(std::max(1, degA(i) - 1)); };
+    for (int a = 0; a < 3; ++a)
+      for (int m = 0; m < Kd; ++m)
+        for (int i = 0; i < K; ++i)
+          if (m >= lim(i)) Dp[(size_t)(a * d.Mp + m) * d.Cp + i] = 0.0;
+    int Mt = (Kd + 15) / 16, Rt = (K + 15) / 16;
+    if (Mt > 16 || Rt > 16) throw std::invalid_argument("order too high for the tile tables");
+    for (int t = 0; t < Mt; ++t) {
+      int kb = K;
+      for (int i = 0; i < K; ++i)
+        if (lim(i) > 16 * t) { kb = i; break; }
+      d.kbeg1[t] = kb & ~3;
+    }
+    for (int t2 = 0; t2 < Rt; ++t2) {
+      int ke = 0;
+      for (int i = 16 * t2; i < std::min(K, 16 * t2 + 16); ++i) ke = std::max(ke, std::min(lim(i), Kd));
+      d.kend2[t2] = (ke + 3) & ~3;
+    }
+  }
   upload(sp->Dp64, Dp, st);
   upload(sp->Dp32, std::vector<float>(Dp.begin(), Dp.end()), st);
   upload(sp->dT32, std::vector<float>(dT.begin(), dT.end()), st);
@@ -273,15 +328,19 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   for (size_t i = 0; i < inv.size(); ++i) inv[i] = 1.0 / sp->diag[i];
   upload(sp->diag_d, sp->diag, st); upload(sp->invdiag, inv, st);
   upload(sp->invdiag32, std::vector<float>(inv.begin(), inv.end()), st);
-  upload(sp->cmask_d, sp->cmask, st);
+  upload_mask(sp.get(), st);
   std::vector<double> xe((size_t)nt * 12);
   for (int e = 0; e < nt; ++e)
     for (int k = 0; k < 4; ++k)
       for (int c = 0; c < 3; ++c) xe[(size_t)e * 12 + 3 * k + c] = xyz[3 * (size_t)tet[4 * e + k] + c];
   upload(sp->xe, xe, st);
   upload(sp->pts3, sp->p3, st);
-  sp->yl64.alloc((size_t)nt * K * sizeof(double));
-  sp->yl32.alloc((size_t)nt * K * sizeof(float));
+  d.ntile = tile_count(nt);
+  d.Kp = (K + 3) & ~3;
+  sp->yl64.alloc((size_t)d.ntile * TILE * K * sizeof(double));
+  sp->yl32.alloc((size_t)d.ntile * TILE * K * sizeof(float));
+  sp->xt64.alloc((size_t)d.ntile * TILE * d.Kp * sizeof(double));
+  sp->xt32.alloc((size_t)d.ntile * TILE * d.Kp * sizeof(float));
   sp->partial.alloc((size_t)4 * (reduce_blocks((int)S.ndof) + 16) * sizeof(double));
   sp->scal.alloc(sizeof(CgScalars));
   d.desc = sp->desc.as<ElemDesc>(); d.slot = sp->slot.as<int32_t>();
@@ -292,8 +351,8 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   d.mT = sp->mT.as<double>(); d.mN = sp->mN.as<double>(); d.mT32 = sp->mT32.as<float>(); d.mN32 = sp->mN32.as<float>();
   d.sc_ptr = sp->sc_ptr.as<int32_t>(); d.sc_code = sp->sc_code.as<int32_t>(); d.sc_w = sp->sc_w.as<double>();
   d.diag = sp->diag_d.as<double>(); d.invdiag = sp->invdiag.as<double>(); d.invdiag32 = sp->invdiag32.as<float>();
-  d.cmask = sp->cmask_d.as<uint8_t>();
   d.Dp64 = sp->Dp64.as<double>(); d.Dp32 = sp->Dp32.as<float>();
+  d.xt64 = sp->xt64.as<double>(); d.xt32 = sp->xt32.as<float>();
   HIPCHK(hipStreamSynchronize(st));
   ctx->nspaces++;
   *out = sp.release();
@@ -479,8 +538,9 @@ int tsem_rhs_begin(tsem_space* sp) {
     }
     upload(sp->wphi, wphi, sp->st());
   }
-  sp->bl.alloc((size_t)S.topo.nt * S.K * sizeof(double));
-  HIPCHK(hipMemsetAsync(sp->bl.p, 0, (size_t)S.topo.nt * S.K * sizeof(double), sp->st()));
+  size_t nbl = (size_t)sp->d.ntile * TILE * S.K;
+  sp->bl.alloc(nbl * sizeof(double));
+  HIPCHK(hipMemsetAsync(sp->bl.p, 0, nbl * sizeof(double), sp->st()));
   HIPCHK(hipStreamSynchronize(sp->st()));
   sp->rhs_open = true;
   API_END
@@ -679,7 +739,7 @@ int tsem_dirichlet(tsem_space* sp, const double* uv, const double* ue, const dou
   for (int64_t i = 0; i < S.ndof; ++i)
     if (!newmask[i] && !sp->cmask[i]) b[i] -= w[i];
   for (int64_t i = 0; i < S.ndof; ++i) sp->cmask[i] |= newmask[i];
-  upload(sp->cmask_d, sp->cmask, st);
+  upload_mask(sp, st);
   HIPCHK(hipMemcpyAsync(b_dev, b.data(), S.ndof * 8, hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
   sp->pmg.reset();
@@ -697,7 +757,7 @@ int tsem_zero_bc(tsem_space* sp, int flag) {
       if (S.mark[dim][g] == flag)
         for (int i = 0; i < S.ndg[dim]; ++i) sp->cmask[S.geo_dof(dim, g) + i] = 1;
   }
-  upload(sp->cmask_d, sp->cmask, sp->st());
+  upload_mask(sp, sp->st());
   HIPCHK(hipStreamSynchronize(sp->st()));
   sp->pmg.reset();
   API_END
@@ -795,15 +855,15 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
   HIPCHK(hipMemcpyAsync(sc, &zero, sizeof(CgScalars), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemsetAsync(p, 0, (size_t)n * sizeof(T), st));
   HIPCHK(hipMemsetAsync(q, 0, (size_t)n * sizeof(T), st));
+  // x += alpha p, r += alpha q, z = P r; partials [r.z | |r|_1 | |r|_2^2] then
+  // beta = r.z / r.z_old in one final reduction
   auto update_and_precondition = [&]() {
     launch_cg_update<T>(d, n, x, r, precond == 4 ? nullptr : z, p, q, sc, part, precond == 4 ? 0 : 1, st);
-    launch_reduce_final(part, nb, 3, sc->tmp, st);
     if (precond == 4) {
       sp->pmg->vcycle<T>(r, z, smooth, st);
-      launch_dot<T>(n, r, z, part, st);
-      launch_reduce_final(part, nb, 1, sc->tmp, st);
+      launch_dot<T>(n, r, z, part, st);  // replaces the r.z partials
     }
-    launch_scalar_beta(sc, st);
+    launch_final_beta(part, nb, 1, sc, st);
   };
   update_and_precondition();
   // p = -z
@@ -822,8 +882,7 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
     int nrun = std::max(1, check_every);
     for (int k = 0; k < nrun && it <= max_iter; ++k) {
       launch_apply<T>(d, op, p, q, yl, part, st);
-      launch_reduce_final(part, reduce_blocks(n), 1, &sc->pq, st);
-      launch_scalar_alpha(sc, st);
+      launch_final_alpha(part, nb, sc, st);
       update_and_precondition();
       launch_p_update<T>(n, p, z, sc, st);
       ++it;

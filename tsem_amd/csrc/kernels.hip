@@ -188,40 +188,66 @@ __global__ void __launch_bounds__(BLK) k_apply(DevSpace S, OpCoef op, const T* _
 }
 
 // ---------------------------------------------------------------- scatter ----
-// y[g] = sum over the transposed map (or diag * x on eliminated DOFs); the
-// optional partial sums of x.y feed the CG denominator.
-template <typename T>
-__global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict__ yl, T* __restrict__ y,
-                                                 const T* __restrict__ xin, int constrained,
-                                                 double* __restrict__ partial) {
-#pragma clang fp contract(off)
-  __shared__ double red[BLK];
-  int g = blockIdx.x * BLK + threadIdx.x;
-  double d = 0.0;
-  if (g < S.ndof) {
-    T acc = T(0);
-    if (constrained && S.cmask[g]) {
-      acc = (T)S.diag[g] * xin[g];
-    } else {
-      for (int t = S.sc_ptr[g]; t < S.sc_ptr[g + 1]; ++t) {
-        int code = S.sc_code[t];
-        T v = yl[code & 0x3fffffff];
-        if (code < 0) v = v * (T)S.sc_w[t];
-        else if (code & 0x40000000) v = -v;
-        acc += v;
-      }
-    }
-    y[g] = acc;
-    if (partial) d = (double)xin[g] * (double)acc;
-  }
-  if (!partial) return;
-  red[threadIdx.x] = d;
+// block reduction of one double per thread (fixed tree)
+__device__ inline double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
   __syncthreads();
   for (int s = BLK / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// one global DOF: fixed-order sum over the transposed gather map (4 loads in
+// flight; the additions stay in map order)
+template <typename T>
+__device__ inline T scatter_one(const DevSpace& S, const T* __restrict__ yl, int g) {
+#pragma clang fp contract(off)
+  int t = S.sc_ptr[g], t1 = S.sc_ptr[g + 1];
+  T acc = T(0);
+  for (; t + 4 <= t1; t += 4) {
+    int c0 = S.sc_code[t], c1 = S.sc_code[t + 1], c2 = S.sc_code[t + 2], c3 = S.sc_code[t + 3];
+    T v0 = yl[c0 & 0x3fffffff], v1 = yl[c1 & 0x3fffffff], v2 = yl[c2 & 0x3fffffff], v3 = yl[c3 & 0x3fffffff];
+    if (c0 < 0) v0 = v0 * (T)S.sc_w[t]; else if (c0 & 0x40000000) v0 = -v0;
+    if (c1 < 0) v1 = v1 * (T)S.sc_w[t + 1]; else if (c1 & 0x40000000) v1 = -v1;
+    if (c2 < 0) v2 = v2 * (T)S.sc_w[t + 2]; else if (c2 & 0x40000000) v2 = -v2;
+    if (c3 < 0) v3 = v3 * (T)S.sc_w[t + 3]; else if (c3 & 0x40000000) v3 = -v3;
+    acc += v0;
+    acc += v1;
+    acc += v2;
+    acc += v3;
+  }
+  for (; t < t1; ++t) {
+    int c0 = S.sc_code[t];
+    T v0 = yl[c0 & 0x3fffffff];
+    if (c0 < 0) v0 = v0 * (T)S.sc_w[t]; else if (c0 & 0x40000000) v0 = -v0;
+    acc += v0;
+  }
+  return acc;
+}
+
+// y[g] = sum over the transposed map (or diag * x on eliminated DOFs); the
+// optional partial sums of x.y feed the CG denominator.  Grid-stride over
+// DOFs with a capped grid (reduce_blocks).
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict__ yl, T* __restrict__ y,
+                                                 const T* __restrict__ xin, int constrained,
+                                                 double* __restrict__ partial) {
+  __shared__ double red[BLK];
+  double d = 0.0;
+  for (int g = blockIdx.x * BLK + threadIdx.x; g < S.ndof; g += gridDim.x * BLK) {
+    T acc;
+    if (constrained && S.cmask[g]) acc = (T)S.diag[g] * xin[g];
+    else acc = scatter_one<T>(S, yl, g);
+    y[g] = acc;
+    if (partial) d += (double)xin[g] * (double)acc;
+  }
+  if (!partial) return;
+  double tot = block_sum(d, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
 // ---- optional per-kernel timing with HIP events on the launching stream ----
@@ -290,7 +316,7 @@ static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T*
     hipLaunchKernelGGL((k_apply<T, NE>), dim3(nb), dim3(BLK), lds, st, S, op, x, yl);
   }
   timing_end(th, st);
-  int ng = (S.ndof + BLK - 1) / BLK;
+  int ng = reduce_blocks(S.ndof);
   th = (op.Kl == 0) ? timing_begin(kind + 1, st) : -1;
   hipLaunchKernelGGL(k_scatter<T>, dim3(ng), dim3(BLK), 0, st, S, yl, y, x, op.constrained, partial);
   timing_end(th, st);
@@ -309,14 +335,18 @@ void launch_apply<float>(const DevSpace& S, const OpCoef& op, const float* x, fl
 
 template <typename T>
 void launch_scatter(const DevSpace& S, const T* yl, T* y, hipStream_t st) {
-  int ng = (S.ndof + BLK - 1) / BLK;
+  int ng = reduce_blocks(S.ndof);
   hipLaunchKernelGGL(k_scatter<T>, dim3(ng), dim3(BLK), 0, st, S, yl, y, (const T*)nullptr, 0, (double*)nullptr);
 }
 template void launch_scatter<double>(const DevSpace&, const double*, double*, hipStream_t);
 template void launch_scatter<float>(const DevSpace&, const float*, float*, hipStream_t);
 
 // ------------------------------------------------------------- reductions ----
-int reduce_blocks(int n) { return (n + BLK - 1) / BLK; }
+// reductions use a capped, n-determined grid with grid-stride loops: the
+// assignment of entries to threads is fixed, so sums are reproducible
+constexpr int RB_MAX = 2048;
+int reduce_blocks(int n) { int b = (n + BLK - 1) / BLK; return b < RB_MAX ? b : RB_MAX; }
+static inline int grid1(int n) { return (n + BLK - 1) / BLK; }
 
 // out[v] = sum_b partial[v * nb + b], fixed tree
 __global__ void __launch_bounds__(BLK) k_reduce_final(const double* __restrict__ partial, int nb, int nvals,
@@ -341,6 +371,44 @@ void launch_reduce_final(const double* partial, int nb, int nvals, double* out, 
 }
 
 // --------------------------------------------------------------- CG steps ----
+// final reductions that also advance the CG scalars
+// alpha = rz / pq
+__global__ void __launch_bounds__(BLK) k_final_alpha(const double* __restrict__ partial, int nb, CgScalars* sc) {
+  __shared__ double red[BLK];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += BLK) s += partial[b];
+  double pq = block_sum(s, red);
+  if (threadIdx.x == 0) {
+    sc->pq = pq;
+    sc->alpha = sc->rz / pq;
+  }
+}
+// tmp[0..2] = r.z, |r|_1, |r|_2^2; beta = r.z_new / r.z
+__global__ void __launch_bounds__(BLK) k_final_beta(const double* __restrict__ partial, int nb, int with_rz,
+                                                    CgScalars* sc) {
+  __shared__ double red[BLK];
+  double v[3];
+  for (int q = 0; q < 3; ++q) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += BLK) s += partial[(size_t)q * nb + b];
+    v[q] = block_sum(s, red);
+  }
+  if (threadIdx.x == 0) {
+    double rz = with_rz ? v[0] : sc->tmp[0];
+    sc->rz_new = rz;
+    sc->beta = rz / sc->rz;
+    sc->rz = rz;
+    sc->l1 = v[1];
+    sc->l2 = v[2];
+  }
+}
+void launch_final_alpha(const double* partial, int nb, CgScalars* sc, hipStream_t st) {
+  hipLaunchKernelGGL(k_final_alpha, dim3(1), dim3(BLK), 0, st, partial, nb, sc);
+}
+void launch_final_beta(const double* partial, int nb, int with_rz, CgScalars* sc, hipStream_t st) {
+  hipLaunchKernelGGL(k_final_beta, dim3(1), dim3(BLK), 0, st, partial, nb, with_rz, sc);
+}
+
 __global__ void k_alpha(CgScalars* sc) { sc->alpha = sc->rz / sc->pq; }
 // tmp[0..2] = r.z, |r|_1, |r|_2^2 of the new residual
 __global__ void k_beta(CgScalars* sc) {
@@ -360,11 +428,10 @@ __global__ void __launch_bounds__(BLK) k_cg_update(DevSpace S, int n, T* __restr
                                                    T* __restrict__ z, const T* __restrict__ p,
                                                    const T* __restrict__ q, const CgScalars* __restrict__ sc,
                                                    double* __restrict__ partial, int jac) {
-  __shared__ double red[3][BLK];
-  int i = blockIdx.x * BLK + threadIdx.x;
+  __shared__ double red[BLK];
   double a = 0, b = 0, c = 0;
-  if (i < n) {
-    T alpha = (T)sc->alpha;
+  const T alpha = (T)sc->alpha;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
     x[i] += alpha * p[i];
     T ri = r[i] + alpha * q[i];
     r[i] = ri;
@@ -376,24 +443,17 @@ __global__ void __launch_bounds__(BLK) k_cg_update(DevSpace S, int n, T* __restr
       zi = ri;
     }
     if (z) z[i] = zi;
-    a = (double)ri * (double)zi;
-    b = fabs((double)ri);
-    c = (double)ri * (double)ri;
+    a += (double)ri * (double)zi;
+    b += fabs((double)ri);
+    c += (double)ri * (double)ri;
   }
-  red[0][threadIdx.x] = a; red[1][threadIdx.x] = b; red[2][threadIdx.x] = c;
-  __syncthreads();
-  for (int s = BLK / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + s];
-      red[1][threadIdx.x] += red[1][threadIdx.x + s];
-      red[2][threadIdx.x] += red[2][threadIdx.x + s];
-    }
-    __syncthreads();
-  }
+  a = block_sum(a, red);
+  b = block_sum(b, red);
+  c = block_sum(c, red);
   if (threadIdx.x == 0) {
-    partial[blockIdx.x] = red[0][0];
-    partial[gridDim.x + blockIdx.x] = red[1][0];
-    partial[2 * gridDim.x + blockIdx.x] = red[2][0];
+    partial[blockIdx.x] = a;
+    partial[gridDim.x + blockIdx.x] = b;
+    partial[2 * gridDim.x + blockIdx.x] = c;
   }
 }
 
@@ -417,7 +477,7 @@ __global__ void __launch_bounds__(BLK) k_p_update(int n, T* __restrict__ p, cons
 }
 template <typename T>
 void launch_p_update(int n, T* p, const T* z, CgScalars* sc, hipStream_t st) {
-  hipLaunchKernelGGL(k_p_update<T>, dim3(reduce_blocks(n)), dim3(BLK), 0, st, n, p, z, sc);
+  hipLaunchKernelGGL(k_p_update<T>, dim3(grid1(n)), dim3(BLK), 0, st, n, p, z, sc);
 }
 template void launch_p_update<double>(int, double*, const double*, CgScalars*, hipStream_t);
 template void launch_p_update<float>(int, float*, const float*, CgScalars*, hipStream_t);
@@ -429,7 +489,7 @@ __global__ void k_axpby(int n, T a, const T* __restrict__ x, T b, T* __restrict_
 }
 template <typename T>
 void launch_axpby(int n, T a, const T* x, T b, T* y, hipStream_t st) {
-  hipLaunchKernelGGL(k_axpby<T>, dim3(reduce_blocks(n)), dim3(BLK), 0, st, n, a, x, b, y);
+  hipLaunchKernelGGL(k_axpby<T>, dim3(grid1(n)), dim3(BLK), 0, st, n, a, x, b, y);
 }
 template void launch_axpby<double>(int, double, const double*, double, double*, hipStream_t);
 template void launch_axpby<float>(int, float, const float*, float, float*, hipStream_t);
@@ -438,14 +498,10 @@ template <typename T>
 __global__ void __launch_bounds__(BLK) k_dot(int n, const T* __restrict__ a, const T* __restrict__ b,
                                              double* __restrict__ partial) {
   __shared__ double red[BLK];
-  int i = blockIdx.x * BLK + threadIdx.x;
-  red[threadIdx.x] = (i < n) ? (double)a[i] * (double)b[i] : 0.0;
-  __syncthreads();
-  for (int s = BLK / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  double s = 0.0;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) s += (double)a[i] * (double)b[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 template <typename T>
 void launch_dot(int n, const T* a, const T* b, double* partial, hipStream_t st) {
@@ -459,35 +515,30 @@ __global__ void k_cast(int n, const double* __restrict__ x, double s, float* __r
   if (i < n) y[i] = (float)(x[i] * s);
 }
 void launch_cast_f64_f32(int n, const double* x, double s, float* y, hipStream_t st) {
-  hipLaunchKernelGGL(k_cast, dim3(reduce_blocks(n)), dim3(BLK), 0, st, n, x, s, y);
+  hipLaunchKernelGGL(k_cast, dim3(grid1(n)), dim3(BLK), 0, st, n, x, s, y);
 }
 __global__ void k_addf(int n, const float* __restrict__ x, double s, double* __restrict__ y) {
   int i = blockIdx.x * BLK + threadIdx.x;
   if (i < n) y[i] += s * (double)x[i];
 }
 void launch_add_f32_f64(int n, const float* x, double s, double* y, hipStream_t st) {
-  hipLaunchKernelGGL(k_addf, dim3(reduce_blocks(n)), dim3(BLK), 0, st, n, x, s, y);
+  hipLaunchKernelGGL(k_addf, dim3(grid1(n)), dim3(BLK), 0, st, n, x, s, y);
 }
 
 // r = b - Ax ; partial |r|_2^2 and |b|_2^2
 __global__ void __launch_bounds__(BLK) k_residual(int n, const double* __restrict__ b, const double* __restrict__ Ax,
                                                   double* __restrict__ r, double* __restrict__ partial) {
-  __shared__ double red[2][BLK];
-  int i = blockIdx.x * BLK + threadIdx.x;
+  __shared__ double red[BLK];
   double a = 0, c = 0;
-  if (i < n) {
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
     double v = b[i] - Ax[i];
     r[i] = v;
-    a = v * v;
-    c = b[i] * b[i];
+    a += v * v;
+    c += b[i] * b[i];
   }
-  red[0][threadIdx.x] = a; red[1][threadIdx.x] = c;
-  __syncthreads();
-  for (int s = BLK / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) { red[0][threadIdx.x] += red[0][threadIdx.x + s]; red[1][threadIdx.x] += red[1][threadIdx.x + s]; }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) { partial[blockIdx.x] = red[0][0]; partial[gridDim.x + blockIdx.x] = red[1][0]; }
+  a = block_sum(a, red);
+  c = block_sum(c, red);
+  if (threadIdx.x == 0) { partial[blockIdx.x] = a; partial[gridDim.x + blockIdx.x] = c; }
 }
 void launch_residual(const DevSpace& S, int n, const double* b, const double* Ax, double* r, double* partial,
                      hipStream_t st) {

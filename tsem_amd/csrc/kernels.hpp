@@ -7,6 +7,16 @@
 
 namespace tsem {
 
+// Gathered element vectors feeding the GEMM live in a tiled layout: tiles of
+// TILE elements, mode-major inside a tile: xt[tile][k][TILE].  Element
+// results are element-major: yl[e][i].
+constexpr int TILE = 32;
+inline size_t tiled_index(int e, int k, int rows) {
+  return (size_t)(e / TILE) * rows * TILE + (size_t)k * TILE + (e % TILE);
+}
+// tiles allocated: whole blocks of 64 elements (the widest GEMM block)
+inline int tile_count(int nt) { return 2 * ((nt + 63) / 64); }
+
 // Everything the kernels read about a space (device pointers).
 struct DevSpace {
   int nt = 0, K = 0, Kd = 0, M = 0, nface = 0;
@@ -35,10 +45,17 @@ struct DevSpace {
   const double* invdiag = nullptr;
   const float* invdiag32 = nullptr;
   const uint8_t* cmask = nullptr;    // 1 where the DOF is eliminated (Dirichlet)
+  const int32_t* gmask = nullptr;    // per element: eliminated geometries (v0..3, e0..5, f0..3, interior)
   // padded derivative factors for the MFMA path: Dp[3 Mp][Cp]
-  int Mp = 0, Cp = 0;
+  int Mp = 0, Cp = 0, Kp = 0, ntile = 0;
   const double* Dp64 = nullptr;
   const float* Dp32 = nullptr;
+  // per 16-row tile: first column of GEMM 1, last row (exclusive) of GEMM 2
+  int kbeg1[16] = {0};
+  int kend2[16] = {0};
+  // tiled scratch for gathered element vectors
+  double* xt64 = nullptr;
+  float* xt32 = nullptr;
 };
 
 // operator selection
@@ -77,6 +94,8 @@ void launch_cg_update(const DevSpace& S, int n, T* x, T* r, T* z, const T* p, co
 template <typename T>
 void launch_p_update(int n, T* p, const T* z, CgScalars* sc, hipStream_t st);
 void launch_scalar_alpha(CgScalars* sc, hipStream_t st);
+void launch_final_alpha(const double* partial, int nb, CgScalars* sc, hipStream_t st);
+void launch_final_beta(const double* partial, int nb, int with_rz, CgScalars* sc, hipStream_t st);
 void launch_scalar_beta(CgScalars* sc, hipStream_t st);
 
 template <typename T>

@@ -98,7 +98,7 @@ Pmg::Pmg(const Space& S, const DevSpace& dd, const std::vector<uint8_t>& cmask, 
                        (double*)inv64[l], (float*)inv32[l]);
     for (int k = 0; k < 4; ++k) HIPCHK(hipMalloc(&work[4 * l + k], (size_t)n * 8));
   }
-  HIPCHK(hipMalloc(&yl, (size_t)d.nt * d.K * 8));
+  HIPCHK(hipMalloc(&yl, (size_t)d.ntile * TILE * d.K * 8));
   // spectral radius of D^-1 A_l by power iteration (FP64)
   lmax.assign(L, 2.0);
   std::mt19937_64 rng(12345);
