@@ -14,7 +14,6 @@ the FP64 matrix-free PCG drop-in is timed on the same system beside it.
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -41,6 +40,24 @@ def cpu_baseline(n=8, p=6, seed_mesh=2026, seed_prob=1234, qorder=None):
                        % (n, len(tet), p, S.ndof, st["iterations"], dt))
 
 
+def profile_traffic(kernel, n, p):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    profiles/hbm_traffic_kuhn<n>_p<p>.md (rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE passes, each scaled by the same counter's reading for a copy of
+    known size); None when no summary covers this workload."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_kuhn%d_p%d.md" % (n, p))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0] == kernel:
+            try:
+                return float(cells[4].replace(",", ""))
+            except ValueError:
+                return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +69,8 @@ def main():
     ap.add_argument("--precond", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="after the solves, copy a buffer of this many MiB three times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1", file=sys.stderr)
@@ -73,11 +92,11 @@ def main():
     b = api.poisson_setup(sp, prob)
     log("rhs + bc %.1f s" % (time.perf_counter() - t1))
 
-    def solve(mixed):
+    def solve(mixed, precond):
         x = sp.zeros()
         torch.cuda.synchronize()
         t = time.perf_counter()
-        x, st = sp.pcg(b, x, tol=1e-10, criterion=1, precond=args.precond, mixed=mixed, inner_tol=1e-4,
+        x, st = sp.pcg(b, x, tol=1e-10, criterion=1, precond=precond, mixed=mixed, inner_tol=1e-4,
                        check_every=20, max_iter=200000)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t
@@ -85,56 +104,74 @@ def main():
             raise RuntimeError("solve did not reach rel-res 1e-10: %s" % st)
         return dt, st
 
-    import ctypes
     from tsem_amd import lib as tl
 
     for _ in range(args.warmup):
-        dt, st = solve(True)
+        dt, st = solve(True, args.precond)
         log("warmup mixed %.3f s %s" % (dt, st))
     times, stats = [], []
     tl.check(tl.load().tsem_timing(1))
     for _ in range(args.steps):
-        dt, st = solve(True)
+        dt, st = solve(True, args.precond)
         times.append(dt)
         stats.append(st)
         log("mixed %.3f s %s" % (dt, st))
-    kt = np.zeros(8)
+    kt = np.zeros(12)
     tl.check(tl.load().tsem_timing_read(tl.dptr(kt)))
     tl.check(tl.load().tsem_timing(0))
-    t64 = []
-    for _ in range(args.fp64_steps):
-        dt, st64 = solve(False)
-        t64.append(dt)
-        log("fp64 %.3f s %s" % (dt, st64))
+    # FP64 drop-in: the reference's PCG on the same system, matrix-free FP64,
+    # with Jacobi (the reference default) and with p-multigrid
+    fp64 = {}
+    for name, pc in (("jacobi", 1), ("pmg", 4)):
+        t64 = []
+        for _ in range(args.fp64_steps):
+            dt, st64 = solve(False, pc)
+            t64.append(dt)
+            log("fp64 %s %.3f s %s" % (name, dt, st64))
+        if t64:
+            fp64[name] = {"value": sp.ndof / float(np.mean(t64)), "ms": float(np.mean(t64)) * 1e3,
+                          "iterations": st64["iterations"]}
+    if args.copy_calib_mib:
+        src = torch.ones(args.copy_calib_mib << 18, dtype=torch.float32, device=sp.device)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: %d MiB read + %d MiB written per copy" % (args.copy_calib_mib, args.copy_calib_mib))
+        del src, dst
     mean = float(np.mean(times))
-    # roofline of the dominant kernel of the timed region (HIP events on the
-    # library stream): element kernel (MFMA, FP32) vs scatter (HBM)
+    # per-kernel HIP-event timing of the mixed solves' full-order FP32 applies
     K, E, N = sp.K, sp.nt, sp.ndof
     Kd = args.p * (args.p + 1) * (args.p + 2) // 6
-    elem_ms = kt[4] / max(kt[5], 1)
-    scat_ms = kt[6] / max(kt[7], 1)
-    flops_elem = 12.0 * Kd * K * E                        # two GEMMs through the derivative factors
+    Kp = (K + 3) // 4 * 4
     nnz_sc = float(sp.nmap)
-    bytes_scat = 4.0 * E * K + 4.0 * nnz_sc + 4.0 * (N + 1) + 4.0 * N + 4.0 * N + 1.0 * N  # yl, codes, ptr, y, x, mask
-    bytes_elem = 4.0 * N + 80.0 * E + 24.0 * E + 4.0 * E * K  # x (unique), descriptors, factors, yl
-    elem_tot, scat_tot = kt[4], kt[6]
-    kernels = {
-        "element_fp32": {"avg_ms": elem_ms, "launches": int(kt[5]), "tflops": flops_elem / (elem_ms * 1e-3) / 1e12 if kt[5] else None,
-                         "algorithmic_flop": flops_elem, "algorithmic_bytes": bytes_elem,
-                         "gbs": bytes_elem / (elem_ms * 1e-3) / 1e9 if kt[5] else None},
-        "scatter_fp32": {"avg_ms": scat_ms, "launches": int(kt[7]), "algorithmic_bytes": bytes_scat,
-                         "gbs": bytes_scat / (scat_ms * 1e-3) / 1e9 if kt[7] else None},
+    algo = {
+        # k_gather_tiles<float>: x once, element descriptors (88 B) + geometry mask, tiled xt written
+        "k_gather_tiles<float>": ("hbm", 4.0 * N + 92.0 * E + 4.0 * E * Kp),
+        # k_elem_gemm<float, 4>: two GEMMs through the derivative factors, 12 Kd K flops per element
+        "k_elem_gemm<float, 4>": ("mfma", 12.0 * Kd * K * E),
+        # k_scatter<float>: element results, map codes + row pointers, y written, x and mask read (fused dot)
+        "k_scatter<float>": ("hbm", 4.0 * E * K + 4.0 * nnz_sc + 4.0 * (N + 1) + 4.0 * N + 4.0 * N + 1.0 * N),
     }
-    if elem_tot >= scat_tot:
-        ach = kernels["element_fp32"]["tflops"]
-        roof = {"kernel": "k_apply_mfma<float> (element operator, FP32 MFMA)", "bound": "mfma", "achieved": ach,
-                "peak": 157.3, "unit": "TFLOP/s", "frac": (ach / 157.3) if ach else None, "traffic": None}
+    kernels = {}
+    for i, name in enumerate(algo):
+        tot, cnt = kt[6 + 2 * i], kt[7 + 2 * i]
+        bound, work = algo[name]
+        avg = tot / cnt if cnt else None
+        rate = work / (avg * 1e-3) / (1e12 if bound == "mfma" else 1e9) if avg else None
+        kernels[name] = {"avg_ms": avg, "launches": int(cnt), "total_ms": tot, "bound": bound,
+                         ("algorithmic_flop" if bound == "mfma" else "algorithmic_bytes"): work,
+                         ("tflops" if bound == "mfma" else "gbs"): rate}
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    kd = kernels[dom]
+    if kd["bound"] == "mfma":
+        roof = {"kernel": dom, "bound": "mfma", "achieved": kd["tflops"], "peak": 157.3, "unit": "TFLOP/s"}
     else:
-        ach = kernels["scatter_fp32"]["gbs"]
-        roof = {"kernel": "k_scatter<float> (deterministic scatter)", "bound": "hbm", "achieved": ach, "peak": 8000.0,
-                "unit": "GB/s", "frac": (ach / 8000.0) if ach else None, "traffic": None}
+        roof = {"kernel": dom, "bound": "hbm", "achieved": kd["gbs"], "peak": 8000.0, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"] if roof["achieved"] else None
+    roof["traffic"] = profile_traffic(dom, args.n, args.p)
     value = sp.ndof / mean
-    fp64_value = sp.ndof / float(np.mean(t64)) if t64 else None
+    fp64_value = max(v["value"] for v in fp64.values()) if fp64 else None
     out = {
         "metric": "solved DOFs/s to rel-res 1e-10, mixed vs FP64 PCG on 1 GPU; p50/p99 latency",
         "value": value,
@@ -156,7 +193,7 @@ def main():
         "mixed_over_fp64": (value / fp64_value) if fp64_value else None,
         "iterations_mixed": stats[-1]["iterations"],
         "outer_mixed": stats[-1]["outer"],
-        "iterations_fp64": st64["iterations"] if t64 else None,
+        "fp64": fp64,
         "relres_mixed": stats[-1]["relres"],
         "roofline": roof,
         "kernels": kernels,

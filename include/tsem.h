@@ -142,9 +142,10 @@ int tsem_scatter(tsem_space* sp, const double* yl_dev, double* y_dev);
 
 /* Kernel timing with HIP events recorded on the launching stream around the
  * full-order operator kernels (process wide).  tsem_timing(1) resets and
- * starts, tsem_timing(0) stops; tsem_timing_read fills out[8] = {ms, count}
- * for: element kernel FP64, scatter FP64, element kernel FP32, scatter FP32
- * (synchronizes the device).  No counterpart. */
+ * starts, tsem_timing(0) stops; tsem_timing_read fills out[12] = {total ms,
+ * launches} for: gather FP64, element GEMM FP64, scatter FP64, gather FP32,
+ * element GEMM FP32, scatter FP32 (the scalar element path counts as element
+ * GEMM; synchronizes the device).  No counterpart. */
 int tsem_timing(int enable);
 int tsem_timing_read(double* out);
 

@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(256) k_gather_tiles(DevSpace S, int Kl, const 
                                                       int constrained) {
   __shared__ ElemDesc sd[TILE];
   __shared__ int sm[TILE];
-  const int tile = blockIdx.x;
+  const int tile = xcd_block(blockIdx.x, gridDim.x);
   const int e0 = tile * TILE;
   {
     const int32_t* src = reinterpret_cast<const int32_t*>(S.desc + e0);
@@ -283,9 +283,15 @@ bool mfma_supported(const DevSpace& S) {
 template <typename T, int CT>
 static void run(const DevSpace& S, const OpCoef& op, const T* Dp, T* xt, const T* x, T* yl, hipStream_t st) {
   const int Kl = op.Kl > 0 ? op.Kl : S.K;
+  const bool timed = op.Kl == 0;
+  const int kind = sizeof(T) == 8 ? TK_GATHER64 : TK_GATHER32;
+  int th = timed ? timing_begin(kind, st) : -1;
   hipLaunchKernelGGL(k_gather_tiles<T>, dim3((unsigned)S.ntile), dim3(256), 0, st, S, Kl, x, xt, op.constrained);
+  timing_end(th, st);
   int nb = (S.nt + 16 * CT - 1) / (16 * CT);
+  th = timed ? timing_begin(kind + 1, st) : -1;
   hipLaunchKernelGGL((k_elem_gemm<T, CT>), dim3(nb), dim3(NT), (gemm_lds<T, CT>(S)), st, S, op, Dp, xt, yl);
+  timing_end(th, st);
 }
 
 template <>

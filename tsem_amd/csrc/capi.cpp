@@ -99,7 +99,7 @@ struct tsem_space {
   DevSpace d;
   // device storage
   DevBuf desc, slot, ft_ptr, ft_key, ft_w, ghat, ghat32, sixv, sixv32, dT, dN, dT32, dN32, mT, mN, mT32, mN32;
-  DevBuf sc_ptr, sc_code, sc_w, diag_d, invdiag, invdiag32, cmask_d, xe, pts3, wphi, Dp64, Dp32;
+  DevBuf sc_ptr, sc_code, sc_w, sc_w32, diag_d, invdiag, invdiag32, cmask_d, xe, pts3, wphi, Dp64, Dp32;
   DevBuf yl64, yl32, bl, partial, scal, xt64, xt32, gmask;
   // solver vectors
   DevBuf v64[6], v32[6];
@@ -324,6 +324,7 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   if (S.sc_ptr.back() >= (1LL << 31)) throw std::invalid_argument("scatter map too large");
   for (size_t i = 0; i < scp.size(); ++i) scp[i] = (int32_t)S.sc_ptr[i];
   upload(sp->sc_ptr, scp, st); upload(sp->sc_code, S.sc_code, st); upload(sp->sc_w, S.sc_w, st);
+  upload(sp->sc_w32, std::vector<float>(S.sc_w.begin(), S.sc_w.end()), st);
   std::vector<double> inv(sp->diag.size());
   for (size_t i = 0; i < inv.size(); ++i) inv[i] = 1.0 / sp->diag[i];
   upload(sp->diag_d, sp->diag, st); upload(sp->invdiag, inv, st);
@@ -349,7 +350,8 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   d.sixv = sp->sixv.as<double>(); d.sixv32 = sp->sixv32.as<float>();
   d.dT = sp->dT.as<double>(); d.dN = sp->dN.as<double>(); d.dT32 = sp->dT32.as<float>(); d.dN32 = sp->dN32.as<float>();
   d.mT = sp->mT.as<double>(); d.mN = sp->mN.as<double>(); d.mT32 = sp->mT32.as<float>(); d.mN32 = sp->mN32.as<float>();
-  d.sc_ptr = sp->sc_ptr.as<int32_t>(); d.sc_code = sp->sc_code.as<int32_t>(); d.sc_w = sp->sc_w.as<double>();
+  d.sc_ptr = sp->sc_ptr.as<int32_t>(); d.sc_code = sp->sc_code.as<int32_t>();
+  d.sc_w = sp->sc_w.as<double>(); d.sc_w32 = sp->sc_w32.as<float>();
   d.diag = sp->diag_d.as<double>(); d.invdiag = sp->invdiag.as<double>(); d.invdiag32 = sp->invdiag32.as<float>();
   d.Dp64 = sp->Dp64.as<double>(); d.Dp32 = sp->Dp32.as<float>();
   d.xt64 = sp->xt64.as<double>(); d.xt32 = sp->xt32.as<float>();

@@ -203,45 +203,52 @@ __device__ inline double block_sum(double v, double* red) {
 
 // one global DOF: fixed-order sum over the transposed gather map (4 loads in
 // flight; the additions stay in map order)
-template <typename T>
-__device__ inline T scatter_one(const DevSpace& S, const T* __restrict__ yl, int g) {
-#pragma clang fp contract(off)
-  int t = S.sc_ptr[g], t1 = S.sc_ptr[g + 1];
-  T acc = T(0);
-  for (; t + 4 <= t1; t += 4) {
-    int c0 = S.sc_code[t], c1 = S.sc_code[t + 1], c2 = S.sc_code[t + 2], c3 = S.sc_code[t + 3];
-    T v0 = yl[c0 & 0x3fffffff], v1 = yl[c1 & 0x3fffffff], v2 = yl[c2 & 0x3fffffff], v3 = yl[c3 & 0x3fffffff];
-    if (c0 < 0) v0 = v0 * (T)S.sc_w[t]; else if (c0 & 0x40000000) v0 = -v0;
-    if (c1 < 0) v1 = v1 * (T)S.sc_w[t + 1]; else if (c1 & 0x40000000) v1 = -v1;
-    if (c2 < 0) v2 = v2 * (T)S.sc_w[t + 2]; else if (c2 & 0x40000000) v2 = -v2;
-    if (c3 < 0) v3 = v3 * (T)S.sc_w[t + 3]; else if (c3 & 0x40000000) v3 = -v3;
-    acc += v0;
-    acc += v1;
-    acc += v2;
-    acc += v3;
-  }
-  for (; t < t1; ++t) {
-    int c0 = S.sc_code[t];
-    T v0 = yl[c0 & 0x3fffffff];
-    if (c0 < 0) v0 = v0 * (T)S.sc_w[t]; else if (c0 & 0x40000000) v0 = -v0;
-    acc += v0;
-  }
-  return acc;
-}
-
-// y[g] = sum over the transposed map (or diag * x on eliminated DOFs); the
-// optional partial sums of x.y feed the CG denominator.  Grid-stride over
-// DOFs with a capped grid (reduce_blocks).
+// y[g] = the row's terms of the transposed gather map summed in map order
+// (or diag * x on eliminated rows).  Rows keep their natural order across
+// lanes, so neighbouring lanes often read the same element-result lines; each
+// lane issues the code loads of up to U terms, then their result loads, then
+// adds in order.  The optional partial sums of x.y (one per block) feed the
+// CG denominator.  The grid is reduce_blocks(ndof) with a fixed row-to-block
+// assignment.
 template <typename T>
 __global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict__ yl, T* __restrict__ y,
                                                  const T* __restrict__ xin, int constrained,
                                                  double* __restrict__ partial) {
+#pragma clang fp contract(off)
   __shared__ double red[BLK];
+  constexpr int U = 8;
+  const T* __restrict__ wt = (sizeof(T) == 8) ? (const T*)S.sc_w : (const T*)S.sc_w32;
   double d = 0.0;
-  for (int g = blockIdx.x * BLK + threadIdx.x; g < S.ndof; g += gridDim.x * BLK) {
-    T acc;
+  // XCD x (blocks b with b % 8 == x) owns rows [x N / 8, (x + 1) N / 8) and
+  // its blocks stride through them together: the rows in flight on one XCD
+  // form one sliding window, so the element results they share stay in that
+  // XCD's L2 (with fewer than 8 blocks the rows are split over the blocks
+  // there are)
+  const int nx = min((int)gridDim.x, NXCD);
+  const int xcd = blockIdx.x % NXCD, jb = blockIdx.x / NXCD;
+  const int nbx = gridDim.x / NXCD + (xcd < (int)(gridDim.x % NXCD) ? 1 : 0);
+  const int g0 = (int)((long long)xcd * S.ndof / nx), g1 = (int)((long long)(xcd + 1) * S.ndof / nx);
+  for (int g = g0 + jb * BLK + threadIdx.x; g < g1; g += nbx * BLK) {
+    const int t0 = S.sc_ptr[g], t1 = S.sc_ptr[g + 1];
+    T acc = T(0);
+    for (int t = t0; t < t1; t += U) {
+      int c[U];
+      T v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) c[u] = (t + u < t1) ? S.sc_code[t + u] : 0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = (t + u < t1) ? yl[c[u] & 0x3fffffff] : T(0);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (t + u < t1) {
+          T w = v[u];
+          if (c[u] < 0) w = w * wt[t + u];
+          else if (c[u] & 0x40000000) w = -w;
+          acc += w;
+        }
+      }
+    }
     if (constrained && S.cmask[g]) acc = (T)S.diag[g] * xin[g];
-    else acc = scatter_one<T>(S, yl, g);
     y[g] = acc;
     if (partial) d += (double)xin[g] * (double)acc;
   }
@@ -271,9 +278,8 @@ void timing_enable(bool on) {
   g_kt.rec.clear();
 }
 
-// kinds: 0 element kernel FP64, 1 scatter FP64, 2 element FP32, 3 scatter FP32
 void timing_read(double* out) {
-  for (int k = 0; k < 8; ++k) out[k] = 0;
+  for (int k = 0; k < 2 * TK_KINDS; ++k) out[k] = 0;
   (void)hipDeviceSynchronize();
   for (auto& r : g_kt.rec) {
     float ms = 0;
@@ -284,7 +290,7 @@ void timing_read(double* out) {
   }
 }
 
-static inline int timing_begin(int kind, hipStream_t st) {
+int timing_begin(int kind, hipStream_t st) {
   if (!g_kt.on || g_kt.used + 2 > g_kt.pool.size()) return -1;
   size_t i = g_kt.used;
   g_kt.used += 2;
@@ -292,7 +298,7 @@ static inline int timing_begin(int kind, hipStream_t st) {
   g_kt.rec.push_back({kind, i});
   return (int)i;
 }
-static inline void timing_end(int h, hipStream_t st) {
+void timing_end(int h, hipStream_t st) {
   if (h >= 0) (void)hipEventRecord(g_kt.pool[h + 1], st);
 }
 
@@ -305,19 +311,19 @@ void set_apply_path(int path) { g_apply_path = path; }
 
 template <typename T>
 static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, double* partial, hipStream_t st) {
-  const int kind = (sizeof(T) == 8) ? 0 : 2;
-  int th = (op.Kl == 0) ? timing_begin(kind, st) : -1;
+  const int kind = (sizeof(T) == 8) ? TK_GATHER64 : TK_GATHER32;
   if (op.mass == 0.0 && g_apply_path == 0 && mfma_supported(S)) {
     launch_apply_mfma<T>(S, op, x, yl, st);
   } else {
     constexpr int NE = 8;
     size_t lds = (size_t)NE * (2 * S.K + 3 * S.Kd) * sizeof(T);
     int nb = (S.nt + NE - 1) / NE;
+    int th = (op.Kl == 0) ? timing_begin(kind + 1, st) : -1;
     hipLaunchKernelGGL((k_apply<T, NE>), dim3(nb), dim3(BLK), lds, st, S, op, x, yl);
+    timing_end(th, st);
   }
-  timing_end(th, st);
   int ng = reduce_blocks(S.ndof);
-  th = (op.Kl == 0) ? timing_begin(kind + 1, st) : -1;
+  int th = (op.Kl == 0) ? timing_begin(kind + 2, st) : -1;
   hipLaunchKernelGGL(k_scatter<T>, dim3(ng), dim3(BLK), 0, st, S, yl, y, x, op.constrained, partial);
   timing_end(th, st);
 }

@@ -11,6 +11,17 @@ namespace tsem {
 // TILE elements, mode-major inside a tile: xt[tile][k][TILE].  Element
 // results are element-major: yl[e][i].
 constexpr int TILE = 32;
+
+// The blocks of one launch are dispatched round-robin over the 8 XCDs (block
+// b runs on XCD b % 8), and each XCD has its own L2.  xcd_block maps the
+// hardware block index to a logical one so that every XCD walks one
+// contiguous range of logical blocks: indirect accesses made by neighbouring
+// work then hit lines already in that XCD's L2.  A bijection on [0, nb).
+constexpr int NXCD = 8;
+__device__ inline int xcd_block(int b, int nb) {
+  int q = nb / NXCD, r = nb % NXCD, x = b % NXCD;
+  return x * q + (x < r ? x : r) + b / NXCD;
+}
 inline size_t tiled_index(int e, int k, int rows) {
   return (size_t)(e / TILE) * rows * TILE + (size_t)k * TILE + (e % TILE);
 }
@@ -39,8 +50,9 @@ struct DevSpace {
   const float* mT32 = nullptr;
   const float* mN32 = nullptr;
   const int32_t* sc_ptr = nullptr;   // ndof + 1
-  const int32_t* sc_code = nullptr;
+  const int32_t* sc_code = nullptr;  // slot | bit30 negative | bit31 weighted
   const double* sc_w = nullptr;
+  const float* sc_w32 = nullptr;
   const double* diag = nullptr;      // assembled operator diagonal
   const double* invdiag = nullptr;
   const float* invdiag32 = nullptr;
@@ -77,8 +89,13 @@ void launch_apply(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, 
 template <typename T>
 void launch_apply_mfma(const DevSpace& S, const OpCoef& op, const T* x, T* yl, hipStream_t st);
 bool mfma_supported(const DevSpace& S);
+// per-kernel HIP-event timing of full-order applies (kinds below); timing_read
+// fills {total ms, launches} per kind
+enum { TK_GATHER64 = 0, TK_ELEM64, TK_SCATTER64, TK_GATHER32, TK_ELEM32, TK_SCATTER32, TK_KINDS };
 void timing_enable(bool on);
 void timing_read(double* out);
+int timing_begin(int kind, hipStream_t st);  // -1 when timing is off
+void timing_end(int handle, hipStream_t st);
 // 0: auto (MFMA when supported), 1: force the scalar element kernel
 void set_apply_path(int path);
 template <typename T>
