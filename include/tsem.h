@@ -162,7 +162,8 @@ typedef struct {
   int max_iter;        /* iteration limit (outer for mixed) */
   int precond;         /* 1: Jacobi (reference default); 4: p-multigrid */
   int mixed;           /* 0: FP64 PCG; 1: FP32 inner PCG inside FP64 iterative refinement */
-  double inner_tol;    /* mixed: relative reduction per inner solve */
+  double inner_tol;    /* mixed: relative reduction per inner solve (the last one stops
+                          once the outer tolerance is in reach, with a margin of 3) */
   int inner_max;       /* mixed: inner iteration limit */
   int smooth_steps;    /* p-MG smoother degree */
   int check_every;     /* host convergence check interval (iterations) */

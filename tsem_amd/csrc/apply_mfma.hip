@@ -84,8 +84,8 @@ __device__ inline T gather_one(const DevSpace& S, int e, int i, const T* __restr
 }  // namespace
 
 // one block per tile of TILE elements: the tile's element descriptors are
-// staged in LDS once, then each thread produces slots (k, element); writes
-// zeros for k >= Kl and for padding elements.  Dirichlet-eliminated DOFs
+// staged in LDS once, then each thread produces slots (k, element) for
+// k < Kl rounded up to 4; writes zeros for k >= Kl and for padding elements.  Dirichlet-eliminated DOFs
 // read as zero through the per-element geometry bits (vertex k: bit k, edge
 // k: bit 4+k, face k: bit 10+k, interior: bit 14).
 template <typename T>
@@ -105,8 +105,9 @@ __global__ void __launch_bounds__(256) k_gather_tiles(DevSpace S, int Kl, const 
   }
   __syncthreads();
   const int Kp = S.Kp;
+  const int Kpl = (Kl + 3) & ~3;  // the GEMM reads rows k < Kpl only
   T* out = xt + (size_t)tile * Kp * TILE;
-  for (int r = threadIdx.x; r < Kp * TILE; r += 256) {
+  for (int r = threadIdx.x; r < Kpl * TILE; r += 256) {
     int k = r / TILE, le = r % TILE, e = e0 + le;
     T v = T(0);
     if (e < S.nt && k < Kl) {

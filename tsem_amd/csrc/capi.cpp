@@ -970,7 +970,10 @@ int tsem_pcg(tsem_space* sp, const double* b, double* x, const tsem_pcg_opts* o,
       HIPCHK(hipMemsetAsync(d32, 0, (size_t)n * 4, st));
       int it = 0;
       double l1 = 0, l2 = 0;
-      pcg_core<float>(sp, r32, d32, o->inner_tol, 1, 1.0, o->inner_max, o->precond, o->smooth_steps,
+      // the inner solve needs no more than the reduction still missing (with
+      // a margin of 3 for the FP32 operator); never less than inner_tol
+      double itol = std::min(0.5, std::max(o->inner_tol, 0.3 * o->tol / rel));
+      pcg_core<float>(sp, r32, d32, itol, 1, 1.0, o->inner_max, o->precond, o->smooth_steps,
                       o->check_every, &it, &l1, &l2);
       total += it;
       launch_add_f32_f64(n, d32, rn, x, st);
