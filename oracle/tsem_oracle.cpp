@@ -23,6 +23,17 @@
 //
 // The mesh topology and the P2 "fem dof" order that AFEPack supplies to the
 // reference are fixed here by a canonical rule (see canonical_topology).
+//
+// Pins (tests/test_oracle.py): multi-index numbering against a table dumped
+// from the reference's own Correspondence header; Jacobi modes against
+// scipy; Gauss-Legendre against numpy; gradients against finite
+// differences; reference matrices by independent quadrature, symmetry,
+// null space and hierarchy; gathered fields continuous across faces; cubic
+// solutions exact; PCG and the p-MG V-cycle against a direct sparse solve.
+// Parity unpinned: AFEPack's absolute geometry numbering (replaced by the
+// canonical rule), the closed-form matrix generators' rounding, deal.II's
+// SpMV summation order, and the Gauss-Seidel V-cycle iterate by iterate
+// (pinned only through the solution it converges to).
 #include "oracle_basis.hpp"
 #include <algorithm>
 #include <cstdio>
