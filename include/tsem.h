@@ -182,6 +182,45 @@ typedef struct {
  * (include/TSEMSolver.h:89-90, src/TSEMSolver.cpp:1775-1930). */
 int tsem_pcg(tsem_space* sp, const double* b_dev, double* x_dev, const tsem_pcg_opts* opts, tsem_pcg_stats* stats);
 
+typedef struct {
+  double tol;          /* criterion 1: max_i ||A x_i - lambda_i B x_i||_2 / (|lambda_i| ||B x_i||_2) <= tol;
+                          criterion 0 (reference): max_i |lambda_i - lambda_i(previous)| < tol */
+  int criterion;
+  int max_iter;        /* iteration limit */
+  int precond;         /* 0: none; 1: Jacobi; 4: one FP32 p-multigrid V-cycle per column */
+  int smooth_steps;    /* p-MG smoother degree */
+  double stiff;        /* A = stiff S + mass M (stiff > 0, mass >= 0); B = M */
+  double mass;
+} tsem_lobpcg_opts;
+
+typedef struct {
+  int iterations;
+  int applies;         /* FP64 operator applies (A and B) */
+  double max_resid;    /* final max relative residual */
+  double max_dlambda;  /* final max eigenvalue change */
+  double seconds;
+} tsem_lobpcg_stats;
+
+/* Block LOBPCG for the m (1..32) lowest eigenpairs of A x = lambda B x on
+ * the eliminated operators (set the Dirichlet marks with tsem_zero_bc
+ * first).  X: device, n_dof x m column-major (column j at X + j n_dof);
+ * in: initial block, out: B-orthonormal eigenvectors, zero on eliminated
+ * DOFs.  lambda: host, m values ascending.  Gram matrices and the
+ * Rayleigh-Ritz problem are FP64 on the full 3m x 3m trial space.
+ * TSEMSolver::solve_LOBPCG(tsem, x, A, B, flag, tol, max_step)
+ * (include/TSEMSolver.h:95-98, src/TSEMSolver.cpp:2224-2407; the reference
+ * solves a fixed 3 x 3 problem for one pair, corrected here). */
+int tsem_lobpcg(tsem_space* sp, int m, double* X_dev, double* lambda, const tsem_lobpcg_opts* opts,
+                tsem_lobpcg_stats* stats);
+
+/* Inverse power iteration for the lowest eigenpair of S x = lambda M x:
+ * x <- S^-1 M x by tsem_pcg with `inner`, M-normalize, stop when the change
+ * of x^T S x is below tol.  x: device, initial guess in, eigenvector out.
+ * TSEMSolver::solve_InversePower(x, A, B, flag, tol, max_step)
+ * (include/TSEMSolver.h:93-94, src/TSEMSolver.cpp:2041-2069). */
+int tsem_inverse_power(tsem_space* sp, double* x_dev, double tol, int max_iter, const tsem_pcg_opts* inner,
+                       double* lambda, int* iterations);
+
 #ifdef __cplusplus
 }
 #endif

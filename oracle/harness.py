@@ -5,6 +5,7 @@ stiffness matrix, calc_rhs(f), Neumann load on mark 2, Dirichlet imposition on
 mark 1 (matrix modified), then solve_PCG.
 """
 import numpy as np
+import scipy.sparse
 
 from oracle import OracleSpace
 from tsem_amd import mesh as meshmod
@@ -55,3 +56,28 @@ def l2_error(S, x, problem):
     u = problem.u(qp)
     vol = S.volumes()
     return float(np.sqrt(np.sum(vol[:, None] * w[None, :] * (uh - u) ** 2)))
+
+
+def dirichlet_pencil(n, M, jitter=0.0, seed=0):
+    """Oracle S and M on the unit cube with zero Dirichlet data on all faces,
+    restricted to the free DOFs (dense).  Returns (S_ff, M_ff, free mask,
+    xyz, tet)."""
+    xyz, tet = meshmod.kuhn_cube(n, jitter=jitter, seed=seed)
+    S = OracleSpace(M, xyz, tet, qorder=M + 3)
+    ev, fv, _, _ = S.topology()
+    mv, me, mf = meshmod.geometry_marks(meshmod.cube_plane_marks(xyz), ev, fv, 0b111111, 0)
+    S.set_marks(mv, me, mf)
+    start = S.dof_start()
+    nv, ne, nf = len(mv), len(me), len(mf)
+    K = [1, M - 1, (M - 1) * (M - 2) // 2]
+    free = np.ones(S.ndof, bool)
+    for d, (marks, off) in enumerate(((mv, 0), (me, nv), (mf, nv + ne))):
+        for g in np.nonzero(marks == 1)[0]:
+            free[start[off + g]:start[off + g] + K[d]] = False
+    mats = []
+    for coef in ((1.0, 0.0), (0.0, 1.0)):
+        S.assemble(*coef)
+        S.zero_bc(1)
+        rs, col, v = S.csr()
+        mats.append(scipy.sparse.csr_matrix((v, col, rs), shape=(S.ndof, S.ndof)).toarray()[np.ix_(free, free)])
+    return mats[0], mats[1], free, xyz, tet

@@ -274,3 +274,16 @@ def test_dirichlet_elimination_structure():
     assert np.abs(off[bnd]).max() == 0 and np.abs(off[:, bnd]).max() == 0
     assert np.abs(A - A.T).max() < 1e-13
     assert np.allclose(b[bnd], np.diag(A)[bnd] * S.bval[bnd])
+
+
+def test_dirichlet_eigenvalues_match_analytic():
+    """Lowest eigenvalues of S x = lambda M x on the unit cube with zero
+    Dirichlet data are pi^2 (a^2 + b^2 + c^2), a, b, c >= 1, with
+    multiplicities: 3 (1), 6 (3), 9 (3), 11 (3), 12 (1), 14 (6)."""
+    import scipy.linalg as la
+    A, B = harness.dirichlet_pencil(3, 6)[:2]
+    w = la.eigh(A, B, eigvals_only=True, subset_by_index=[0, 16])
+    exact = np.pi ** 2 * np.array([3] + [6] * 3 + [9] * 3 + [11] * 3 + [12] + [14] * 6, float)
+    # spectral accuracy at p = 6 on 3^3 cells: relative errors about 1e-5
+    assert np.all(np.abs(w - exact) <= 5e-5 * exact), w / np.pi ** 2
+    assert np.all(w >= exact * (1 - 1e-12))  # conforming Galerkin: eigenvalues from above

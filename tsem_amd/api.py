@@ -239,6 +239,41 @@ class Space:
                        converged=(rc == L.TSEM_OK))
 
 
+    def lobpcg(self, m, X=None, seed=7, tol=1e-8, criterion=1, max_iter=1000, precond=4, smooth_steps=2,
+               stiff=1.0, mass=0.0, raise_on_noconv=True):
+        """solve_LOBPCG: the m lowest eigenpairs of (stiff S + mass M) x = lambda M x
+        on the eliminated operators.  X: (m, n_dof) float64 device tensor
+        (row j = vector j), default N(0, 1) from `seed`.  Returns
+        (lambda (numpy, ascending), X, stats dict)."""
+        import torch
+        if X is None:
+            g = torch.Generator(device="cpu").manual_seed(seed)
+            X = torch.randn((m, self.ndof), generator=g, dtype=torch.float64).to(self.device)
+        X = X.contiguous()
+        lam = np.zeros(m)
+        o = L.LobpcgOpts(tol, criterion, max_iter, precond, smooth_steps, stiff, mass)
+        s = L.LobpcgStats()
+        _ready()
+        rc = L.load().tsem_lobpcg(self.h, m, _ptr(X), L.dptr(lam), C.byref(o), C.byref(s))
+        if rc != L.TSEM_OK and (rc != L.TSEM_ENOCONV or raise_on_noconv):
+            L.check(rc)
+        return lam, X, dict(iterations=s.iterations, applies=s.applies, max_resid=s.max_resid,
+                            max_dlambda=s.max_dlambda, seconds=s.seconds, converged=(rc == L.TSEM_OK))
+
+    def inverse_power(self, x=None, seed=7, tol=1e-12, max_iter=1000, inner_tol=1e-14, precond=1):
+        """solve_InversePower: lowest eigenpair of S x = lambda M x (eliminated)."""
+        import torch
+        if x is None:
+            g = torch.Generator(device="cpu").manual_seed(seed)
+            x = torch.rand(self.ndof, generator=g, dtype=torch.float64).to(self.device)
+        o = L.PcgOpts(inner_tol, 1, 100000, precond, 0, 1e-4, 100000, 2, 20)
+        lam = np.zeros(1)
+        its = C.c_int()
+        _ready()
+        L.check(L.load().tsem_inverse_power(self.h, _ptr(x), tol, max_iter, C.byref(o), L.dptr(lam), C.byref(its)))
+        return float(lam[0]), x, its.value
+
+
 def poisson_setup(sp, problem, dirichlet_planes=0b000001, neumann_planes=0b111110):
     """Mixed-BC Poisson system on the unit cube, the sequence of
     examples/poisson_mixBC/poisson.cpp:74-89: load vector, Neumann load on

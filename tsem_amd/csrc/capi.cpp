@@ -16,47 +16,17 @@
 #include <string>
 #include <vector>
 
-#include "kernels.hpp"
-#include "pmg.hpp"
-#include "space.hpp"
+#include "handle.hpp"
 
 using namespace tsem;
 
-static thread_local std::string g_err;
+namespace tsem {
+thread_local std::string g_err;
+}
 
 const char* tsem_last_error(void) { return g_err.c_str(); }
 
-#define HIPCHK(x)                                                                      \
-  do {                                                                                 \
-    hipError_t e_ = (x);                                                               \
-    if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP: ") + hipGetErrorString(e_) + " at " #x); \
-  } while (0)
-
 namespace {
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() { if (p) (void)hipFree(p); }
-  void alloc(size_t bytes) {
-    if (p && n >= bytes) return;
-    if (p) { (void)hipFree(p); p = nullptr; }
-    if (bytes == 0) bytes = 8;
-    if (hipMalloc(&p, bytes) != hipSuccess) { p = nullptr; throw std::bad_alloc(); }
-    n = bytes;
-  }
-  template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
-};
-
-template <typename T>
-void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
-  b.alloc(v.size() * sizeof(T));
-  if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
-}
-
 // standard Jacobi P_n^{(a,b)}(x), a,b > -1
 double jacobi(double a, double b, int n, double x) {
   if (n == 0) return 1.0;
@@ -75,62 +45,10 @@ double jacobi(double a, double b, int n, double x) {
 
 }  // namespace
 
-// A context stays alive until its last space is destroyed, whatever order
-// the caller releases them in.
-struct tsem_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int nspaces = 0;
-  bool closing = false;
-};
-
 static void ctx_release(tsem_ctx* c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
-
-struct tsem_space {
-  tsem_ctx* ctx = nullptr;
-  Space h;
-  int qorder = 0, nq1 = 0, nq2 = 0, nq3 = 0;
-  std::vector<double> t1, w1, p2, w2, p3, w3;  // BC/RHS quadrature
-  std::vector<double> diag;                     // host copy
-  std::vector<uint8_t> cmask;
-  DevSpace d;
-  // device storage
-  DevBuf desc, slot, ft_ptr, ft_key, ft_w, ghat, ghat32, sixv, sixv32, dT, dN, dT32, dN32, mT, mN, mT32, mN32;
-  DevBuf sc_ptr, sc_code, sc_w, sc_w32, diag_d, invdiag, invdiag32, cmask_d, xe, pts3, wphi, Dp64, Dp32;
-  DevBuf yl64, yl32, bl, partial, scal, xt64, xt32, gmask;
-  // solver vectors
-  DevBuf v64[6], v32[6];
-  std::unique_ptr<Pmg> pmg;
-  bool rhs_open = false;
-  hipStream_t st() const {
-    if (!ctx) throw std::logic_error("space was created without a device context");
-    return ctx->stream;
-  }
-};
-
-#define API_BEGIN try {
-#define API_END                                         \
-  }                                                     \
-  catch (const std::bad_alloc&) {                       \
-    g_err = "allocation failed";                        \
-    return TSEM_ENOMEM;                                 \
-  }                                                     \
-  catch (const std::invalid_argument& e) {              \
-    g_err = e.what();                                   \
-    return TSEM_EINVAL;                                 \
-  }                                                     \
-  catch (const std::logic_error& e) {                   \
-    g_err = e.what();                                   \
-    return TSEM_ESTATE;                                 \
-  }                                                     \
-  catch (const std::exception& e) {                     \
-    g_err = e.what();                                   \
-    return TSEM_EDEVICE;                                \
-  }                                                     \
-  return TSEM_OK;
 
 // the face table ships next to the library: <dir of libtsem_amd.so>/data/face_table.txt
 static std::string face_table_path() {

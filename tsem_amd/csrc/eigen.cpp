@@ -1,0 +1,417 @@
+// Eigen solvers on the device operators of a space: block LOBPCG and inverse
+// power iteration for A x = lambda B x with A = stiff S + mass M and B = M,
+// both with the space's Dirichlet elimination (vectors kept zero on the
+// eliminated DOFs, so the eliminated rows add no spurious eigenpairs).
+//
+// LOBPCG follows the block method of TSEMSolver::solve_LOBPCG
+// (src/TSEMSolver.cpp:2224-2407): trial block [X W P], Gram matrices of A
+// and B, Rayleigh-Ritz, X and P recombined from the trial block, stop on the
+// change of the eigenvalues.  Corrected here: the reference solves a fixed
+// 3 x 3 problem for one pair (dsygvx with N = 3, il = iu = 1, :2349) and
+// leaves the other eigenvalues unset; this solves the full 3m x 3m problem
+// for all m pairs.  W is the preconditioned residual (Jacobi, or one FP32
+// p-multigrid V-cycle per column) instead of an inner PCG solve of the
+// rebuilt matrix S/2 - lambda M (:2292-2323).  For stability near
+// convergence, converged pairs leave W (soft locking), W is B-orthogonalized
+// against X and P, and W and P are B-orthonormalized with dependent
+// directions dropped.  Gram matrices, the Rayleigh-Ritz problem and the
+// recombinations are FP64.
+//
+// Inverse power follows solve_InversePower (src/TSEMSolver.cpp:2041-2069):
+// x <- A^-1 B x by PCG, B-normalize, e = x^T A x, stop when |e - e_last| < tol.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <numeric>
+#include <vector>
+
+#include "handle.hpp"
+
+using namespace tsem;
+
+namespace {
+
+// Cyclic Jacobi eigenvalue method for a symmetric n x n matrix (row-major):
+// eigenvalues ascending in w, eigenvectors in the columns of V (row-major).
+void sym_eig(int n, std::vector<double> A, std::vector<double>& w, std::vector<double>& V) {
+  std::vector<double> Q((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < n; ++i) {
+      diag += A[(size_t)i * n + i] * A[(size_t)i * n + i];
+      for (int j = i + 1; j < n; ++j) off += A[(size_t)i * n + j] * A[(size_t)i * n + j];
+    }
+    if (off <= 1e-32 * diag || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double apq = A[(size_t)p * n + q];
+        if (apq == 0.0) continue;
+        double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        // rotation that zeroes A[p][q]: tan(phi) = t, the smaller root of
+        // t^2 + 2 theta t - 1 = 0 with theta = (aqq - app) / (2 apq)
+        double theta = (aqq - app) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {  // columns p, q
+          double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+          A[(size_t)k * n + p] = c * akp - s * akq;
+          A[(size_t)k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // rows p, q
+          double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+          A[(size_t)p * n + k] = c * apk - s * aqk;
+          A[(size_t)q * n + k] = s * apk + c * aqk;
+        }
+        A[(size_t)p * n + q] = A[(size_t)q * n + p] = 0.0;
+        for (int k = 0; k < n; ++k) {
+          double qkp = Q[(size_t)k * n + p], qkq = Q[(size_t)k * n + q];
+          Q[(size_t)k * n + p] = c * qkp - s * qkq;
+          Q[(size_t)k * n + q] = s * qkp + c * qkq;
+        }
+      }
+  }
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return A[(size_t)a * n + a] < A[(size_t)b * n + b]; });
+  w.resize(n);
+  V.assign((size_t)n * n, 0.0);
+  for (int c = 0; c < n; ++c) {
+    w[c] = A[(size_t)order[c] * n + order[c]];
+    for (int k = 0; k < n; ++k) V[(size_t)k * n + c] = Q[(size_t)k * n + order[c]];
+  }
+}
+
+// Rayleigh-Ritz on the trial space: the m lowest eigenpairs of
+// GA c = lambda GB c (p x p, symmetric; GB positive semidefinite).  GB is
+// equilibrated by its diagonal and directions whose relative GB-eigenvalue
+// is below `drop` are removed, so a nearly dependent trial block is handled
+// as a pseudo-inverse.  Returns C (p x m, row-major, GB-orthonormal columns)
+// and the eigenvalues; false when fewer than m directions remain.
+bool rayleigh_ritz(int p, std::vector<double> GA, std::vector<double> GB, int m, double drop, std::vector<double>& C,
+                   std::vector<double>& lam) {
+  for (int i = 0; i < p; ++i)
+    for (int j = i + 1; j < p; ++j) {
+      double a = 0.5 * (GA[(size_t)i * p + j] + GA[(size_t)j * p + i]);
+      double b = 0.5 * (GB[(size_t)i * p + j] + GB[(size_t)j * p + i]);
+      GA[(size_t)i * p + j] = GA[(size_t)j * p + i] = a;
+      GB[(size_t)i * p + j] = GB[(size_t)j * p + i] = b;
+    }
+  std::vector<double> d(p);
+  for (int i = 0; i < p; ++i) {
+    double g = GB[(size_t)i * p + i];
+    d[i] = g > 0 ? 1.0 / std::sqrt(g) : 0.0;
+  }
+  for (int i = 0; i < p; ++i)
+    for (int j = 0; j < p; ++j) {
+      GB[(size_t)i * p + j] *= d[i] * d[j];
+      GA[(size_t)i * p + j] *= d[i] * d[j];
+    }
+  std::vector<double> wb, Vb;
+  sym_eig(p, GB, wb, Vb);
+  double wmax = wb.empty() ? 0.0 : wb.back();
+  std::vector<int> keep;
+  for (int k = 0; k < p; ++k)
+    if (wb[k] > drop * wmax) keep.push_back(k);
+  int r = (int)keep.size();
+  if (r < m) return false;
+  // Z = Vb[:, keep] / sqrt(wb): GB-orthonormal basis of the kept directions
+  std::vector<double> Z((size_t)p * r);
+  for (int i = 0; i < p; ++i)
+    for (int c = 0; c < r; ++c) Z[(size_t)i * r + c] = Vb[(size_t)i * p + keep[c]] / std::sqrt(wb[keep[c]]);
+  // H = Z^T GA Z
+  std::vector<double> T((size_t)p * r, 0.0), H((size_t)r * r, 0.0);
+  for (int i = 0; i < p; ++i)
+    for (int k = 0; k < p; ++k) {
+      double g = GA[(size_t)i * p + k];
+      if (g == 0.0) continue;
+      for (int c = 0; c < r; ++c) T[(size_t)i * r + c] += g * Z[(size_t)k * r + c];
+    }
+  for (int a = 0; a < r; ++a)
+    for (int i = 0; i < p; ++i) {
+      double z = Z[(size_t)i * r + a];
+      if (z == 0.0) continue;
+      for (int c = 0; c < r; ++c) H[(size_t)a * r + c] += z * T[(size_t)i * r + c];
+    }
+  std::vector<double> wh, Vh;
+  sym_eig(r, H, wh, Vh);
+  C.assign((size_t)p * m, 0.0);
+  lam.assign(wh.begin(), wh.begin() + m);
+  for (int i = 0; i < p; ++i)
+    for (int j = 0; j < m; ++j) {
+      double s = 0.0;
+      for (int c = 0; c < r; ++c) s += Z[(size_t)i * r + c] * Vh[(size_t)c * r + j];
+      C[(size_t)i * m + j] = s * d[i];
+    }
+  return true;
+}
+
+// Orthonormalization of a block U against G = U^T B U (k x k): returns T
+// (k x k2, row-major) with (U T)^T B (U T) = I, dropping directions whose
+// relative eigenvalue of the diagonally equilibrated G is below tau.
+int svqb(int k, std::vector<double> G, double tau, std::vector<double>& T) {
+  std::vector<double> d(k);
+  for (int i = 0; i < k; ++i) {
+    double g = G[(size_t)i * k + i];
+    d[i] = g > 0 ? 1.0 / std::sqrt(g) : 0.0;
+  }
+  std::vector<double> H((size_t)k * k);
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) H[(size_t)i * k + j] = 0.5 * (G[(size_t)i * k + j] + G[(size_t)j * k + i]) * d[i] * d[j];
+  std::vector<double> th, V;
+  sym_eig(k, H, th, V);
+  double tmax = th.empty() ? 0.0 : th.back();
+  std::vector<int> keep;
+  for (int c = 0; c < k; ++c)
+    if (th[c] > tau * tmax && th[c] > 0) keep.push_back(c);
+  int k2 = (int)keep.size();
+  T.assign((size_t)k * k2, 0.0);
+  for (int i = 0; i < k; ++i)
+    for (int c = 0; c < k2; ++c) T[(size_t)i * k2 + c] = d[i] * V[(size_t)i * k + keep[c]] / std::sqrt(th[keep[c]]);
+  return k2;
+}
+
+}  // namespace
+
+namespace tsem {
+
+// y = A x for each of m columns (n-strided), FP64, eliminated form
+static void apply_block(tsem_space* sp, double stiff, double mass, const double* X, double* Y, int m) {
+  OpCoef op;
+  op.stiff = stiff;
+  op.mass = mass;
+  op.constrained = 1;
+  const int n = sp->d.ndof;
+  for (int j = 0; j < m; ++j)
+    launch_apply<double>(sp->d, op, X + (size_t)j * n, Y + (size_t)j * n, sp->yl64.as<double>(), nullptr, sp->st());
+}
+
+}  // namespace tsem
+
+extern "C" {
+
+int tsem_lobpcg(tsem_space* sp, int m, double* X, double* lambda, const tsem_lobpcg_opts* o,
+                tsem_lobpcg_stats* stats) {
+  API_BEGIN
+  if (!sp || !X || !lambda || !o) throw std::invalid_argument("null argument");
+  if (m < 1 || m > 32) throw std::invalid_argument("block size must be in 1..32");
+  if (!(o->tol > 0)) throw std::invalid_argument("tol must be positive");
+  if (o->precond != 0 && o->precond != 1 && o->precond != 4)
+    throw std::invalid_argument("precond must be 0 (none), 1 (Jacobi) or 4 (p-MG)");
+  if (o->criterion != 0 && o->criterion != 1) throw std::invalid_argument("criterion must be 0 or 1");
+  if (!(o->stiff > 0) || o->mass < 0) throw std::invalid_argument("A = stiff S + mass M needs stiff > 0, mass >= 0");
+  hipStream_t st = sp->st();
+  const int n = sp->d.ndof;
+  if (n < 3 * m) throw std::invalid_argument("block too large for the space");
+  auto t0 = std::chrono::steady_clock::now();
+  if (o->precond == 4 && !sp->pmg) sp->pmg.reset(new Pmg(sp->h, sp->d, sp->cmask, st));
+  const size_t nm = (size_t)n * m;
+  // trial block [X | P | W] and its images under A and B: X at column 0, P
+  // (np columns) at column m, W (nw columns: residuals of the pairs not yet
+  // converged, preconditioned, B-orthogonalized against X and P and
+  // orthonormalized) right after P
+  DevBuf S, AS, BS, R, tmp, part, gram, dlam, dC, r32, z32;
+  S.alloc(3 * nm * 8); AS.alloc(3 * nm * 8); BS.alloc(3 * nm * 8);
+  R.alloc(nm * 8);
+  tmp.alloc(2 * nm * 8);
+  part.alloc((size_t)gram_blocks(n) * 96 * 96 * 8);
+  gram.alloc(96 * 96 * 8);
+  dlam.alloc(32 * 8);
+  dC.alloc(96 * 32 * 8);
+  double* base[3] = {S.as<double>(), AS.as<double>(), BS.as<double>()};
+  double* Sx = base[0];
+  double* Ax = base[1];
+  double* Bx = base[2];
+  double* T = tmp.as<double>();
+  const uint8_t* cm = sp->d.cmask;
+  int applies = 0;
+
+  auto col = [&](int b, int c) { return base[b] + (size_t)c * n; };
+  auto gram_host = [&](int p, int q, const double* U, const double* V, std::vector<double>& G) {
+    G.assign((size_t)p * q, 0.0);
+    if (p < 1 || q < 1) return;
+    launch_gram(n, p, q, U, n, V, n, part.as<double>(), gram.as<double>(), st);
+    HIPCHK(hipMemcpyAsync(G.data(), gram.p, (size_t)p * q * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  };
+  auto upload_coef = [&](const std::vector<double>& C) {
+    HIPCHK(hipMemcpyAsync(dC.p, C.data(), C.size() * 8, hipMemcpyHostToDevice, st));
+    return dC.as<double>();
+  };
+  // columns [c0, c0 + k) of S, AS, BS <- the same columns times Tm (k x k2)
+  auto transform = [&](int c0, int k, int k2, const std::vector<double>& Tm) {
+    const double* dt = upload_coef(Tm);
+    for (int b = 0; b < 3; ++b) {
+      launch_combine(n, k, k2, col(b, c0), n, dt, T, n, 1.0, 0.0, st);
+      if (k2 > 0) HIPCHK(hipMemcpyAsync(col(b, c0), T, (size_t)n * k2 * 8, hipMemcpyDeviceToDevice, st));
+    }
+  };
+  // columns [w0, w0 + nw) -= columns [c0, c0 + k) Cm, Cm = ([c0..]^T B [w0..]) (k x nw)
+  auto orthogonalize = [&](int c0, int k, int w0, int nw) {
+    if (k < 1 || nw < 1) return;
+    std::vector<double> Cm;
+    gram_host(k, nw, col(0, c0), col(2, w0), Cm);
+    const double* dc = upload_coef(Cm);
+    for (int b = 0; b < 3; ++b) launch_combine(n, k, nw, col(b, c0), n, dc, col(b, w0), n, -1.0, 1.0, st);
+  };
+
+  // initial block: zero the eliminated rows, Rayleigh-Ritz on X alone
+  HIPCHK(hipMemcpyAsync(Sx, X, nm * 8, hipMemcpyDeviceToDevice, st));
+  launch_block_mask(n, m, cm, nullptr, Sx, st);
+  apply_block(sp, o->stiff, o->mass, Sx, Ax, m);
+  apply_block(sp, 0.0, 1.0, Sx, Bx, m);
+  applies += 2 * m;
+  std::vector<double> GA, GB, C, lam, lam_last, RR, BB, Tm;
+  gram_host(m, m, Sx, Ax, GA);
+  gram_host(m, m, Sx, Bx, GB);
+  if (!rayleigh_ritz(m, GA, GB, m, 1e-14, C, lam)) throw std::invalid_argument("initial block is rank deficient");
+  transform(0, m, m, C);
+  int np = 0, it = 0;
+  double maxres = 0.0, maxdl = 0.0;
+  bool conv = false;
+  static const bool trace = getenv("TSEM_LOBPCG_TRACE") != nullptr;
+  for (; it < o->max_iter; ++it) {
+    // residuals and their relative norms
+    HIPCHK(hipMemcpyAsync(dlam.p, lam.data(), (size_t)m * 8, hipMemcpyHostToDevice, st));
+    launch_block_resid(n, m, Ax, Bx, dlam.as<double>(), cm, R.as<double>(), st);
+    gram_host(m, m, R.as<double>(), R.as<double>(), RR);
+    gram_host(m, m, Bx, Bx, BB);
+    maxres = 0.0;
+    std::vector<double> res(m);
+    for (int j = 0; j < m; ++j) {
+      res[j] = std::sqrt(std::max(0.0, RR[(size_t)j * m + j])) /
+               (std::fabs(lam[j]) * std::sqrt(std::max(1e-300, BB[(size_t)j * m + j])));
+      maxres = std::max(maxres, res[j]);
+    }
+    maxdl = 0.0;
+    if (!lam_last.empty())
+      for (int j = 0; j < m; ++j) maxdl = std::max(maxdl, std::fabs(lam[j] - lam_last[j]));
+    if (trace) fprintf(stderr, "[tsem lobpcg] it %d maxres %.3e maxdl %.3e np %d lam0 %.12e\n", it, maxres, maxdl, np,
+                       lam[0]);
+    if (o->criterion == 1 ? maxres <= o->tol : (!lam_last.empty() && maxdl < o->tol)) {
+      conv = true;
+      break;
+    }
+    lam_last = lam;
+    // W: residuals of the pairs not converged yet (soft locking)
+    const int w0 = m + np;
+    int nw = 0;
+    std::vector<double> wnorm;
+    for (int j = 0; j < m; ++j) {
+      if (o->criterion == 1 && res[j] <= o->tol) continue;
+      HIPCHK(hipMemcpyAsync(col(0, w0 + nw), R.as<double>() + (size_t)j * n, (size_t)n * 8, hipMemcpyDeviceToDevice,
+                            st));
+      wnorm.push_back(std::sqrt(std::max(0.0, RR[(size_t)j * m + j])));
+      ++nw;
+    }
+    double* Wp = col(0, w0);
+    if (o->precond == 1) {
+      launch_block_mask(n, nw, cm, sp->d.invdiag, Wp, st);
+    } else if (o->precond == 4) {
+      r32.alloc((size_t)n * 4);
+      z32.alloc((size_t)n * 4);
+      for (int k = 0; k < nw; ++k) {
+        double nr = wnorm[k];
+        if (!(nr > 0)) continue;
+        double* w = Wp + (size_t)k * n;
+        launch_cast_f64_f32(n, w, 1.0 / nr, r32.as<float>(), st);
+        sp->pmg->vcycle<float>(r32.as<float>(), z32.as<float>(), o->smooth_steps, st);
+        HIPCHK(hipMemsetAsync(w, 0, (size_t)n * 8, st));
+        launch_add_f32_f64(n, z32.as<float>(), nr, w, st);
+      }
+      launch_block_mask(n, nw, cm, nullptr, Wp, st);
+    }
+    apply_block(sp, o->stiff, o->mass, Wp, col(1, w0), nw);
+    apply_block(sp, 0.0, 1.0, Wp, col(2, w0), nw);
+    applies += 2 * nw;
+    // B-orthogonalize W against X and P, then orthonormalize it
+    orthogonalize(0, m, w0, nw);
+    orthogonalize(m, np, w0, nw);
+    if (nw > 0) {
+      gram_host(nw, nw, Wp, col(2, w0), GB);
+      int nw2 = svqb(nw, GB, 1e-12, Tm);
+      transform(w0, nw, nw2, Tm);
+      nw = nw2;
+    }
+    // Rayleigh-Ritz on [X P W]
+    int p = m + np + nw;
+    gram_host(p, p, Sx, Ax, GA);
+    gram_host(p, p, Sx, Bx, GB);
+    if (!rayleigh_ritz(p, GA, GB, m, 1e-13, C, lam)) throw std::runtime_error("LOBPCG trial space collapsed");
+    // X <- [X P W] C, P <- [P W] C[m:], then orthonormalize P
+    const double* dc = upload_coef(C);
+    for (int b = 0; b < 3; ++b) {
+      launch_combine(n, p, m, base[b], n, dc, T, n, 1.0, 0.0, st);
+      if (p > m) launch_combine(n, p - m, m, col(b, m), n, dc + (size_t)m * m, T + nm, n, 1.0, 0.0, st);
+      HIPCHK(hipMemcpyAsync(base[b], T, nm * 8, hipMemcpyDeviceToDevice, st));
+      if (p > m) HIPCHK(hipMemcpyAsync(col(b, m), T + nm, nm * 8, hipMemcpyDeviceToDevice, st));
+    }
+    np = 0;
+    if (p > m) {
+      gram_host(m, m, col(0, m), col(2, m), GB);
+      np = svqb(m, GB, 1e-12, Tm);
+      transform(m, m, np, Tm);
+    }
+  }
+  HIPCHK(hipMemcpyAsync(X, Sx, nm * 8, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int j = 0; j < m; ++j) lambda[j] = lam[j];
+  if (stats) {
+    stats->iterations = it;
+    stats->max_resid = maxres;
+    stats->max_dlambda = maxdl;
+    stats->applies = applies;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (!conv) { g_err = "LOBPCG did not reach the tolerance"; return TSEM_ENOCONV; }
+  API_END
+}
+
+int tsem_inverse_power(tsem_space* sp, double* x, double tol, int max_iter, const tsem_pcg_opts* inner,
+                       double* lambda, int* iterations) {
+  API_BEGIN
+  if (!sp || !x || !inner || !lambda) throw std::invalid_argument("null argument");
+  if (!(tol > 0)) throw std::invalid_argument("tol must be positive");
+  hipStream_t st = sp->st();
+  const int n = sp->d.ndof;
+  DevBuf rhs, ax, part, g;
+  rhs.alloc((size_t)n * 8); ax.alloc((size_t)n * 8);
+  part.alloc((size_t)gram_blocks(n) * 8); g.alloc(8);
+  auto dot = [&](const double* a, const double* b) {
+    double h = 0;
+    launch_gram(n, 1, 1, a, n, b, n, part.as<double>(), g.as<double>(), st);
+    HIPCHK(hipMemcpyAsync(&h, g.p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return h;
+  };
+  launch_block_mask(n, 1, sp->d.cmask, nullptr, x, st);
+  apply_block(sp, 0.0, 1.0, x, rhs.as<double>(), 1);
+  double e_last = 0, e_now = 0;
+  int it = 0;
+  bool conv = false;
+  for (; it < max_iter; ++it) {
+    tsem_pcg_stats s;
+    int rc = tsem_pcg(sp, rhs.as<double>(), x, inner, &s);
+    if (rc != TSEM_OK && rc != TSEM_ENOCONV) return rc;
+    apply_block(sp, 0.0, 1.0, x, rhs.as<double>(), 1);
+    double c = std::sqrt(dot(rhs.as<double>(), x));
+    launch_axpby<double>(n, 1.0 / c, x, 0.0, x, st);
+    launch_axpby<double>(n, 1.0 / c, rhs.as<double>(), 0.0, rhs.as<double>(), st);
+    apply_block(sp, 1.0, 0.0, x, ax.as<double>(), 1);
+    e_now = dot(ax.as<double>(), x);
+    if (it > 0 && std::fabs(e_now - e_last) < tol) { conv = true; ++it; break; }
+    e_last = e_now;
+  }
+  *lambda = e_now;
+  if (iterations) *iterations = it;
+  if (!conv) { g_err = "inverse power iteration did not reach the tolerance"; return TSEM_ENOCONV; }
+  API_END
+}
+
+}  // extern "C"

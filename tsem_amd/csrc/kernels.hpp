@@ -128,4 +128,19 @@ void launch_rhs_local(const DevSpace& S, int e0, int e1, int nq, const double* w
                       const double* fq, double* bl, hipStream_t st);
 void launch_qp_coords(const DevSpace& S, const double* xyz_elem /*nt x 12*/, int e0, int e1, int nq,
                       const double* pts /*nq x 3*/, double* out, hipStream_t st);
+
+// ---- block vectors (eigen solvers); column j of a block starts at j * ld
+int gram_blocks(int n);  // partial buffer: gram_blocks(n) * p * q doubles
+// out[i q + j] = sum_r U[r][i] V[r][j] (p, q <= 96), fixed-order sums
+void launch_gram(int n, int p, int q, const double* U, long ldu, const double* V, long ldv, double* partial,
+                 double* out, hipStream_t st);
+// Y = alpha S C + beta Y with C row-major p x m on device (p <= 96, m <= 32)
+void launch_combine(int n, int p, int m, const double* S, long lds, const double* C, double* Y, long ldy,
+                    double alpha, double beta, hipStream_t st);
+// R[:, j] = AX[:, j] - lam[j] BX[:, j] (lam on device), zero on eliminated rows
+void launch_block_resid(int n, int m, const double* AX, const double* BX, const double* lam, const uint8_t* cmask,
+                        double* R, hipStream_t st);
+// zero eliminated rows of an n x m block; scale rows by inv when not null
+void launch_block_mask(int n, int m, const uint8_t* cmask, const double* inv, double* X, hipStream_t st);
+
 }  // namespace tsem

@@ -29,6 +29,16 @@ class PcgStats(C.Structure):
                 ("seconds", C.c_double)]
 
 
+class LobpcgOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("criterion", C.c_int), ("max_iter", C.c_int), ("precond", C.c_int),
+                ("smooth_steps", C.c_int), ("stiff", C.c_double), ("mass", C.c_double)]
+
+
+class LobpcgStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("applies", C.c_int), ("max_resid", C.c_double),
+                ("max_dlambda", C.c_double), ("seconds", C.c_double)]
+
+
 # exported entry points: name -> argument types (all return int status unless noted)
 _vp, _ip, _dp, _lp = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_int64)
 SIGNATURES = {
@@ -62,6 +72,8 @@ SIGNATURES = {
     "tsem_timing": [C.c_int],
     "tsem_timing_read": [_dp],
     "tsem_pcg": [_vp, _vp, _vp, C.POINTER(PcgOpts), C.POINTER(PcgStats)],
+    "tsem_lobpcg": [_vp, C.c_int, _vp, _dp, C.POINTER(LobpcgOpts), C.POINTER(LobpcgStats)],
+    "tsem_inverse_power": [_vp, _vp, C.c_double, C.c_int, C.POINTER(PcgOpts), _dp, C.POINTER(C.c_int)],
 }
 
 _lib = None
