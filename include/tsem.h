@@ -189,8 +189,9 @@ typedef struct {
   int max_iter;        /* iteration limit */
   int precond;         /* 0: none; 1: Jacobi; 4: one FP32 p-multigrid V-cycle per column */
   int smooth_steps;    /* p-MG smoother degree */
-  double stiff;        /* A = stiff S + mass M (stiff > 0, mass >= 0); B = M */
+  double stiff;        /* A = stiff S + mass M + potential M_V (stiff > 0); B = M */
   double mass;
+  double potential;    /* coefficient of M_V (tsem_set_potential); 0 = none */
 } tsem_lobpcg_opts;
 
 typedef struct {
@@ -220,6 +221,46 @@ int tsem_lobpcg(tsem_space* sp, int m, double* X_dev, double* lambda, const tsem
  * (include/TSEMSolver.h:93-94, src/TSEMSolver.cpp:2041-2069). */
 int tsem_inverse_power(tsem_space* sp, double* x_dev, double tol, int max_iter, const tsem_pcg_opts* inner,
                        double* lambda, int* iterations);
+
+/* ---- work at the 3-D quadrature points (n_qp3 points per element, weights
+ * summing to 1 over the element, tsem_qp_coords gives the points) ---- */
+
+/* u_h at the points of elements [e0, e1): out[(e - e0) n_qp3 + q] (device).
+ * TSEM::calc_val_qp_onElement (src/PointInfo.cpp:60-91). */
+int tsem_qp_values(tsem_space* sp, const double* x_dev, int e0, int e1, double* out_dev);
+
+/* rho[e][q] = sum_k occ[k] psi_k(x_q)^2 over all elements (device, nt x
+ * n_qp3); psi: device, n_dof x nk column-major; occ: host, nk values.
+ * TSEM::calc_density (src/PointInfo.cpp:2012-2022). */
+int tsem_density(tsem_space* sp, int nk, const double* psi_dev, const double* occ, double* rho_dev);
+
+/* out[v] = sum_e vol_e sum_q w_q f g m_v(x_q) over all elements, for the
+ * moments m = {1, x, y, z, xx, xy, xz, yy, yz, zz} (nmom = 1 or 10); f, g:
+ * device, nt x n_qp3 (g may be NULL).  Used for the energies and the
+ * multipole data of the Kohn-Sham example (examples/KohnSham-Helium/
+ * SCFIteration.cpp:70-101, 303-326).  Fixed-order sums. */
+int tsem_integrate(tsem_space* sp, const double* f_dev, const double* g_dev, int nmom, double* out);
+
+/* sum over elements [e0, e1) of vol_e sum_q w_q (u_h(x_q) - u_q)^2, u: device
+ * values at the points of those elements; the L2 error is the square root
+ * of the sum over all chunks.  TSEM::calc_l2_error (src/CalcError.cpp:20-126). */
+int tsem_l2_error_chunk(tsem_space* sp, const double* x_dev, int e0, int e1, const double* u_dev, double* sum);
+
+/* LDA (unpolarized) at n points: Slater exchange + Perdew-Zunger 1981
+ * correlation; vxc = d(rho eps_xc)/d rho, exc = rho eps_xc (device; either
+ * output may be NULL).  The reference example uses VWN correlation
+ * (examples/KohnSham-Helium/SCFIteration.cpp:192-230). */
+int tsem_lda_pz(tsem_space* sp, int64_t n, const double* rho_dev, double* vxc_dev, double* exc_dev);
+
+/* Potential V at the points (device, nt x n_qp3, copied; NULL clears) that
+ * weights the operator M_V, (M_V)_ij = int V phi_i phi_j, applied
+ * matrix-free (TSEM::build_mass_V_matrix, src/Discretize.cpp:667-843;
+ * needs qorder <= 20). */
+int tsem_set_potential(tsem_space* sp, const double* v_dev);
+
+/* y = (stiff S + mass M + potential M_V) x, FP64 device vectors. */
+int tsem_apply_potential(tsem_space* sp, double stiff, double mass, double potential, int constrained,
+                         const double* x_dev, double* y_dev);
 
 #ifdef __cplusplus
 }

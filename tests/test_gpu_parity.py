@@ -140,3 +140,20 @@ def test_pmg_matches_jacobi_solution(ctx):
     assert sm["iterations"] < sj["iterations"] / 3
     a, c = xj.cpu().numpy(), xm.cpu().numpy()
     assert np.linalg.norm(a - c) <= 1e-10 * np.linalg.norm(a)
+
+
+def test_dirichlet_reimposed_after_zero_elimination(ctx):
+    """Data imposed on DOFs that an earlier call already eliminated (the
+    Hartree solve of every SCF step) give the same system as on a fresh space."""
+    prob = problems.Manufactured(1234)
+    xyz, tet = mesh.kuhn_cube(2, jitter=0.1, seed=12)
+    a = api.Space(ctx, 4, xyz, tet, qorder=8)
+    b = api.Space(ctx, 4, xyz, tet, qorder=8)
+    ba = api.poisson_setup(a, prob)
+    b.set_cube_marks()
+    b.zero_bc(1)
+    bb = api.poisson_setup(b, prob)
+    assert np.abs(ba.cpu().numpy() - bb.cpu().numpy()).max() <= 1e-13 * np.abs(ba.cpu().numpy()).max()
+    xa, _ = a.pcg(ba, tol=1e-12)
+    xb, _ = b.pcg(bb, tol=1e-12)
+    assert np.abs(xa.cpu().numpy() - xb.cpu().numpy()).max() <= 1e-11 * np.abs(xa.cpu().numpy()).max()

@@ -133,9 +133,11 @@ class Space:
         self.marks = [np.ascontiguousarray(m, dtype=np.int32) for m in (mv, me, mf)]
         L.check(L.load().tsem_set_marks(self.h, *(L.iptr(m) for m in self.marks)))
 
-    def set_cube_marks(self, dirichlet_planes=0b000001, neumann_planes=0b111110):
+    def set_cube_marks(self, dirichlet_planes=0b000001, neumann_planes=0b111110, lo=0.0, hi=1.0):
+        """Marks of an axis-aligned box [lo, hi]^3: 1 on the Dirichlet planes,
+        2 on the Neumann planes (bits x=lo, x=hi, y=lo, y=hi, z=lo, z=hi)."""
         ev, fv, _, _ = self.topology()
-        mv, me, mf = geometry_marks(cube_plane_marks(self.xyz), ev, fv, dirichlet_planes, neumann_planes)
+        mv, me, mf = geometry_marks(cube_plane_marks(self.xyz, lo, hi), ev, fv, dirichlet_planes, neumann_planes)
         self.set_marks(mv, me, mf)
         return mv, me, mf
 
@@ -180,6 +182,97 @@ class Space:
         _ready()
         L.check(lib.tsem_rhs_end(self.h, _ptr(b)))
         return b
+
+    def rhs_qp(self, fq):
+        """calc_rhs from values at the 3-D quadrature points of all elements
+        (device, nt x n_qp3)."""
+        b = self.zeros()
+        lib = L.load()
+        fq = fq.contiguous()
+        _ready()
+        L.check(lib.tsem_rhs_begin(self.h))
+        L.check(lib.tsem_rhs_add(self.h, 0, self.nt, _ptr(fq)))
+        L.check(lib.tsem_rhs_end(self.h, _ptr(b)))
+        return b
+
+    def qp_coords(self, e0=0, e1=None):
+        """Physical coordinates of the 3-D quadrature points, (e1-e0, n_qp3, 3)."""
+        import torch
+        e1 = self.nt if e1 is None else e1
+        pts = torch.empty((e1 - e0, self.nq3, 3), dtype=torch.float64, device=self.device)
+        L.check(L.load().tsem_qp_coords(self.h, e0, e1, _ptr(pts)))
+        return pts
+
+    def qp_values(self, x, e0=0, e1=None):
+        """calc_val_qp_onElement: u_h at the quadrature points, (e1-e0, n_qp3)."""
+        import torch
+        e1 = self.nt if e1 is None else e1
+        out = torch.empty((e1 - e0, self.nq3), dtype=torch.float64, device=self.device)
+        _ready()
+        L.check(L.load().tsem_qp_values(self.h, _ptr(x), e0, e1, _ptr(out)))
+        return out
+
+    def density(self, psi, occ):
+        """calc_density: sum_k occ_k psi_k^2 at the quadrature points; psi (k, n_dof)."""
+        import torch
+        psi = psi.contiguous()
+        occ = np.ascontiguousarray(occ, dtype=np.float64)
+        rho = torch.empty((self.nt, self.nq3), dtype=torch.float64, device=self.device)
+        _ready()
+        L.check(L.load().tsem_density(self.h, len(occ), _ptr(psi), L.dptr(occ), _ptr(rho)))
+        return rho
+
+    def integrate(self, f, g=None, moments=False):
+        """sum_e vol_e sum_q w_q f g (and its moments 1, x, y, z, xx, xy, xz,
+        yy, yz, zz when moments=True) over all elements."""
+        out = np.zeros(10)
+        f = f.contiguous()
+        g = g.contiguous() if g is not None else None
+        _ready()
+        L.check(L.load().tsem_integrate(self.h, _ptr(f), _ptr(g) if g is not None else None, 10 if moments else 1,
+                                        L.dptr(out)))
+        return out if moments else float(out[0])
+
+    def l2_error(self, x, u, chunk=None):
+        """calc_l2_error: ||u_h - u|| with u a function of the points (device)."""
+        import math
+        if chunk is None:
+            chunk = max(1, min(self.nt, int(2e8 // max(1, self.nq3 * 3))))
+        tot = 0.0
+        s = C.c_double()
+        for e0 in range(0, self.nt, chunk):
+            e1 = min(self.nt, e0 + chunk)
+            uq = u(self.qp_coords(e0, e1)).contiguous()
+            _ready()
+            L.check(L.load().tsem_l2_error_chunk(self.h, _ptr(x), e0, e1, _ptr(uq), C.byref(s)))
+            tot += s.value
+        return math.sqrt(tot)
+
+    def lda_pz(self, rho):
+        """LDA exchange (Slater) + Perdew-Zunger correlation: (v_xc, e_xc) at the points."""
+        import torch
+        rho = rho.contiguous()
+        v = torch.empty_like(rho)
+        e = torch.empty_like(rho)
+        _ready()
+        L.check(L.load().tsem_lda_pz(self.h, rho.numel(), _ptr(rho), _ptr(v), _ptr(e)))
+        return v, e
+
+    def set_potential(self, v):
+        """Potential at the quadrature points (nt x n_qp3) weighting M_V; None clears."""
+        if v is not None:
+            v = v.contiguous()
+        _ready()
+        L.check(L.load().tsem_set_potential(self.h, _ptr(v) if v is not None else None))
+
+    def apply_op(self, x, stiff=1.0, mass=0.0, potential=0.0, constrained=False):
+        """y = (stiff S + mass M + potential M_V) x (FP64)."""
+        import torch
+        y = torch.empty_like(x)
+        _ready()
+        L.check(L.load().tsem_apply_potential(self.h, stiff, mass, potential, 1 if constrained else 0, _ptr(x),
+                                              _ptr(y)))
+        return y
 
     def neumann(self, g_face_qp, flag, b):
         g = np.ascontiguousarray(g_face_qp, dtype=np.float64)
@@ -240,9 +333,9 @@ class Space:
 
 
     def lobpcg(self, m, X=None, seed=7, tol=1e-8, criterion=1, max_iter=1000, precond=4, smooth_steps=2,
-               stiff=1.0, mass=0.0, raise_on_noconv=True):
-        """solve_LOBPCG: the m lowest eigenpairs of (stiff S + mass M) x = lambda M x
-        on the eliminated operators.  X: (m, n_dof) float64 device tensor
+               stiff=1.0, mass=0.0, potential=0.0, raise_on_noconv=True):
+        """solve_LOBPCG: the m lowest eigenpairs of (stiff S + mass M + potential M_V) x
+        = lambda M x on the eliminated operators (M_V: set_potential).  X: (m, n_dof) float64 device tensor
         (row j = vector j), default N(0, 1) from `seed`.  Returns
         (lambda (numpy, ascending), X, stats dict)."""
         import torch
@@ -251,7 +344,7 @@ class Space:
             X = torch.randn((m, self.ndof), generator=g, dtype=torch.float64).to(self.device)
         X = X.contiguous()
         lam = np.zeros(m)
-        o = L.LobpcgOpts(tol, criterion, max_iter, precond, smooth_steps, stiff, mass)
+        o = L.LobpcgOpts(tol, criterion, max_iter, precond, smooth_steps, stiff, mass, potential)
         s = L.LobpcgStats()
         _ready()
         rc = L.load().tsem_lobpcg(self.h, m, _ptr(X), L.dptr(lam), C.byref(o), C.byref(s))

@@ -648,7 +648,13 @@ int tsem_dirichlet(tsem_space* sp, const double* uv, const double* ue, const dou
       }
     }
   }
-  // free rows: b -= A ext(val), with the elimination imposed so far
+  // free rows: b -= A ext(val), with the elimination imposed so far on the
+  // other DOFs (the DOFs imposed here are unmasked for this product, so data
+  // imposed again on DOFs eliminated before still reach the free rows)
+  std::vector<uint8_t> prev(sp->cmask);
+  for (int64_t i = 0; i < S.ndof; ++i)
+    if (newmask[i]) sp->cmask[i] = 0;
+  upload_mask(sp, st);
   DevBuf tmpv, tmpw;
   upload(tmpv, vb, st);
   tmpw.alloc(S.ndof * 8);
@@ -658,7 +664,7 @@ int tsem_dirichlet(tsem_space* sp, const double* uv, const double* ue, const dou
   HIPCHK(hipStreamSynchronize(st));
   for (int64_t i = 0; i < S.ndof; ++i)
     if (!newmask[i] && !sp->cmask[i]) b[i] -= w[i];
-  for (int64_t i = 0; i < S.ndof; ++i) sp->cmask[i] |= newmask[i];
+  for (int64_t i = 0; i < S.ndof; ++i) sp->cmask[i] = prev[i] | newmask[i];
   upload_mask(sp, st);
   HIPCHK(hipMemcpyAsync(b_dev, b.data(), S.ndof * 8, hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));

@@ -1,5 +1,6 @@
 // Eigen solvers on the device operators of a space: block LOBPCG and inverse
-// power iteration for A x = lambda B x with A = stiff S + mass M and B = M,
+// power iteration for A x = lambda B x with A = stiff S + mass M (+ potential
+// M_V, the mass matrix weighted by the potential set on the space) and B = M,
 // both with the space's Dirichlet elimination (vectors kept zero on the
 // eliminated DOFs, so the eliminated rows add no spurious eigenpairs).
 //
@@ -182,10 +183,12 @@ int svqb(int k, std::vector<double> G, double tau, std::vector<double>& T) {
 namespace tsem {
 
 // y = A x for each of m columns (n-strided), FP64, eliminated form
-static void apply_block(tsem_space* sp, double stiff, double mass, const double* X, double* Y, int m) {
+static void apply_block(tsem_space* sp, double stiff, double mass, const double* X, double* Y, int m,
+                        double pot = 0.0) {
   OpCoef op;
   op.stiff = stiff;
   op.mass = mass;
+  op.pot = pot;
   op.constrained = 1;
   const int n = sp->d.ndof;
   for (int j = 0; j < m; ++j)
@@ -205,7 +208,8 @@ int tsem_lobpcg(tsem_space* sp, int m, double* X, double* lambda, const tsem_lob
   if (o->precond != 0 && o->precond != 1 && o->precond != 4)
     throw std::invalid_argument("precond must be 0 (none), 1 (Jacobi) or 4 (p-MG)");
   if (o->criterion != 0 && o->criterion != 1) throw std::invalid_argument("criterion must be 0 or 1");
-  if (!(o->stiff > 0) || o->mass < 0) throw std::invalid_argument("A = stiff S + mass M needs stiff > 0, mass >= 0");
+  if (!(o->stiff > 0)) throw std::invalid_argument("A = stiff S + mass M + potential M_V needs stiff > 0");
+  if (o->potential != 0.0 && !sp->d.vpot) throw std::logic_error("potential coefficient set but no potential");
   hipStream_t st = sp->st();
   const int n = sp->d.ndof;
   if (n < 3 * m) throw std::invalid_argument("block too large for the space");
@@ -264,7 +268,7 @@ int tsem_lobpcg(tsem_space* sp, int m, double* X, double* lambda, const tsem_lob
   // initial block: zero the eliminated rows, Rayleigh-Ritz on X alone
   HIPCHK(hipMemcpyAsync(Sx, X, nm * 8, hipMemcpyDeviceToDevice, st));
   launch_block_mask(n, m, cm, nullptr, Sx, st);
-  apply_block(sp, o->stiff, o->mass, Sx, Ax, m);
+  apply_block(sp, o->stiff, o->mass, Sx, Ax, m, o->potential);
   apply_block(sp, 0.0, 1.0, Sx, Bx, m);
   applies += 2 * m;
   std::vector<double> GA, GB, C, lam, lam_last, RR, BB, Tm;
@@ -327,7 +331,7 @@ int tsem_lobpcg(tsem_space* sp, int m, double* X, double* lambda, const tsem_lob
       }
       launch_block_mask(n, nw, cm, nullptr, Wp, st);
     }
-    apply_block(sp, o->stiff, o->mass, Wp, col(1, w0), nw);
+    apply_block(sp, o->stiff, o->mass, Wp, col(1, w0), nw, o->potential);
     apply_block(sp, 0.0, 1.0, Wp, col(2, w0), nw);
     applies += 2 * nw;
     // B-orthogonalize W against X and P, then orthonormalize it

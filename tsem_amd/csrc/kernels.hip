@@ -11,6 +11,8 @@
 //  CG/vector: fused PCG updates with deterministic two-stage reductions.
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include <cstdlib>
 #include <utility>
 #include <vector>
@@ -78,6 +80,34 @@ __global__ void __launch_bounds__(BLK) k_gather(DevSpace S, const double* __rest
 void launch_gather(const DevSpace& S, const double* x, double* xl, int constrained, hipStream_t st) {
   long n = (long)S.nt * S.K;
   hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + BLK - 1) / BLK)), dim3(BLK), 0, st, S, x, xl, constrained);
+}
+
+// yl[e][i] += pot vol_e sum_q w_q V[e][q] phi_i(x_q) sum_j phi_j(x_q) x_e[j]:
+// the potential mass operator M_V applied matrix-free (the assembled form is
+// TSEM::build_mass_V_matrix, src/Discretize.cpp:667-843).  One block per
+// element; the element coefficients and the weighted point values sit in LDS.
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_apply_potential(DevSpace S, double pot, const T* __restrict__ x,
+                                                         T* __restrict__ yl, int constrained) {
+  extern __shared__ double psm[];
+  double* xe = psm;         // K
+  double* c = psm + S.K;    // nq
+  const int e = blockIdx.x, K = S.K, nq = S.nq;
+  for (int i = threadIdx.x; i < K; i += BLK) xe[i] = (double)gather_slot<T>(S, e, i, x, constrained);
+  __syncthreads();
+  const double scale = pot * S.sixv[e] / 6.0;
+  const double* V = S.vpot + (size_t)e * nq;
+  for (int q = threadIdx.x; q < nq; q += BLK) {
+    double u = 0.0;
+    for (int i = 0; i < K; ++i) u += S.phiT[(size_t)i * nq + q] * xe[i];
+    c[q] = scale * S.wq[q] * V[q] * u;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < K; i += BLK) {
+    double y = 0.0;
+    for (int q = 0; q < nq; ++q) y += S.phi[(size_t)q * K + i] * c[q];
+    yl[(size_t)e * K + i] += (T)y;
+  }
 }
 
 // ----------------------------------------------------------------- apply ----
@@ -321,6 +351,12 @@ static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T*
     int th = (op.Kl == 0) ? timing_begin(kind + 1, st) : -1;
     hipLaunchKernelGGL((k_apply<T, NE>), dim3(nb), dim3(BLK), lds, st, S, op, x, yl);
     timing_end(th, st);
+  }
+  if (op.pot != 0.0) {
+    if (!S.vpot) throw std::logic_error("potential operator requested but no potential is set");
+    size_t lds = (size_t)(S.K + S.nq) * sizeof(double);
+    if (lds > 64 * 1024) throw std::invalid_argument("quadrature too fine for the potential operator (qorder <= 20)");
+    hipLaunchKernelGGL(k_apply_potential<T>, dim3(S.nt), dim3(BLK), lds, st, S, op.pot, x, yl, op.constrained);
   }
   int ng = reduce_blocks(S.ndof);
   int th = (op.Kl == 0) ? timing_begin(kind + 2, st) : -1;

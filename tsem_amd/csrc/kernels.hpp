@@ -57,7 +57,18 @@ struct DevSpace {
   const double* invdiag = nullptr;
   const float* invdiag32 = nullptr;
   const uint8_t* cmask = nullptr;    // 1 where the DOF is eliminated (Dirichlet)
-  const int32_t* gmask = nullptr;    // per element: eliminated geometries (v0..3, e0..5, f0..3, interior)
+  const int32_t* gmask = nullptr;
+  // 3-D quadrature of the space (load vector, point values, M_V): nq points,
+  // weights wq (sum 1 over the reference tet), phi[q][i], phiT[i][q]; vpot:
+  // potential at the points, [nt][nq] (M_V, set by the caller); xe: vertex
+  // coordinates [nt][4][3]; pts: reference coordinates of the points [nq][3]
+  int nq = 0;
+  const double* wq = nullptr;
+  const double* phi = nullptr;
+  const double* phiT = nullptr;
+  const double* vpot = nullptr;
+  const double* xe = nullptr;
+  const double* pts = nullptr;    // per element: eliminated geometries (v0..3, e0..5, f0..3, interior)
   // padded derivative factors for the MFMA path: Dp[3 Mp][Cp]
   int Mp = 0, Cp = 0, Kp = 0, ntile = 0;
   const double* Dp64 = nullptr;
@@ -73,6 +84,7 @@ struct DevSpace {
 // operator selection
 struct OpCoef {
   double stiff = 1.0, mass = 0.0;
+  double pot = 0.0;      // coefficient of the potential mass operator M_V (DevSpace::vpot)
   int constrained = 0;   // honour the Dirichlet elimination
   int Kl = 0, Kdl = 0;   // leading modes used (p-multigrid levels); 0 = all
 };
@@ -142,5 +154,23 @@ void launch_block_resid(int n, int m, const double* AX, const double* BX, const 
                         double* R, hipStream_t st);
 // zero eliminated rows of an n x m block; scale rows by inv when not null
 void launch_block_mask(int n, int m, const uint8_t* cmask, const double* inv, double* X, hipStream_t st);
+
+
+// ---- quadrature-point work (values, density, integrals, LDA)
+// out[(e - e0) nq + q] = sum_i phi_i(x_q) xl[e][i] for e in [e0, e1)
+void launch_qp_values(const DevSpace& S, int e0, int e1, const double* xl, double* out, hipStream_t st);
+// rho[e][q] = sum_k occ[k] (sum_i phi_i(x_q) xl_k[e][i])^2, xl_k = xl + k nt K
+void launch_density(const DevSpace& S, int nk, const double* xl, const double* occ_dev, double* rho, hipStream_t st);
+int integrate_blocks(const DevSpace& S);
+// out[v] = sum_e vol_e sum_q w_q f[e][q] m_v(x_q), m = {1, x, y, z, xx, xy, xz, yy, yz, zz}
+// (v < nmom; with g != null, f g instead of f), fixed-order sums
+void launch_integrate(const DevSpace& S, const double* f, const double* g, int nmom, double* partial, double* out,
+                      hipStream_t st);
+// out = sum_{e in [e0, e1)} vol_e sum_q w_q (u_h(x_q) - u[(e - e0) nq + q])^2
+void launch_l2_chunk(const DevSpace& S, int e0, int e1, const double* xl, const double* u, double* partial,
+                     double* out, hipStream_t st);
+// LDA, Slater exchange + Perdew-Zunger correlation (unpolarized): potential
+// v_xc and energy density e_xc = rho (eps_x + eps_c) per point
+void launch_lda_pz(long n, const double* rho, double* vxc, double* exc, hipStream_t st);
 
 }  // namespace tsem

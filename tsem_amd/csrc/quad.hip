@@ -1,0 +1,205 @@
+// Work at the 3-D quadrature points of a space: values of a DOF vector
+// (TSEM::calc_val_qp_onElement, src/PointInfo.cpp:60-91), the density of
+// occupied states (TSEM::calc_density, src/PointInfo.cpp:2012-2022),
+// weighted integrals and low-order moments, the L2 error
+// (TSEM::calc_l2_error, src/CalcError.cpp:20-126), and LDA exchange and
+// correlation.  Point values of an element come from its gathered
+// coefficients: u(x_q) = sum_i phi_i(x_q) xl[e][i].  Integrals over an
+// element are vol_e sum_q w_q (.), the weights summing to 1.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "kernels.hpp"
+
+namespace tsem {
+
+namespace {
+constexpr int QB = 256;
+constexpr int IB = 512;  // max blocks of the integral kernels
+constexpr int MOM = 10;  // moments {1, x, y, z, xx, xy, xz, yy, yz, zz}
+
+__device__ inline double block_reduce(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int k = QB / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ inline void point(const DevSpace& S, int e, int q, double* x) {
+  const double* P = S.xe + (size_t)e * 12;
+  double a = S.pts[3 * q], b = S.pts[3 * q + 1], c = S.pts[3 * q + 2], l0 = 1 - a - b - c;
+  for (int d = 0; d < 3; ++d) x[d] = l0 * P[d] + a * P[3 + d] + b * P[6 + d] + c * P[9 + d];
+}
+}  // namespace
+
+// one block per element, threads over the points
+__global__ void __launch_bounds__(QB) k_qp_values(DevSpace S, int e0, const double* __restrict__ xl,
+                                                  double* __restrict__ out) {
+  extern __shared__ double xs[];
+  const int e = e0 + blockIdx.x, K = S.K, nq = S.nq;
+  for (int i = threadIdx.x; i < K; i += QB) xs[i] = xl[(size_t)e * K + i];
+  __syncthreads();
+  for (int q = threadIdx.x; q < nq; q += QB) {
+    double u = 0.0;
+    for (int i = 0; i < K; ++i) u += S.phiT[(size_t)i * nq + q] * xs[i];
+    out[(size_t)blockIdx.x * nq + q] = u;
+  }
+}
+
+void launch_qp_values(const DevSpace& S, int e0, int e1, const double* xl, double* out, hipStream_t st) {
+  if (e1 <= e0) return;
+  hipLaunchKernelGGL(k_qp_values, dim3(e1 - e0), dim3(QB), (size_t)S.K * 8, st, S, e0, xl, out);
+}
+
+__global__ void __launch_bounds__(QB) k_density(DevSpace S, int nk, const double* __restrict__ xl,
+                                                const double* __restrict__ occ, double* __restrict__ rho) {
+  extern __shared__ double xs[];  // nk x K
+  const int e = blockIdx.x, K = S.K, nq = S.nq;
+  for (int s = threadIdx.x; s < nk * K; s += QB) {
+    int k = s / K, i = s % K;
+    xs[s] = xl[((size_t)k * S.nt + e) * K + i];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nq; q += QB) {
+    double r = 0.0;
+    for (int k = 0; k < nk; ++k) {
+      double u = 0.0;
+      for (int i = 0; i < K; ++i) u += S.phiT[(size_t)i * nq + q] * xs[k * K + i];
+      r += occ[k] * u * u;
+    }
+    rho[(size_t)e * nq + q] = r;
+  }
+}
+
+void launch_density(const DevSpace& S, int nk, const double* xl, const double* occ, double* rho, hipStream_t st) {
+  hipLaunchKernelGGL(k_density, dim3(S.nt), dim3(QB), (size_t)nk * S.K * 8, st, S, nk, xl, occ, rho);
+}
+
+int integrate_blocks(const DevSpace& S) { return S.nt < IB ? S.nt : IB; }
+
+// partial[v * nb + b]: block b sums a fixed contiguous range of elements
+__global__ void __launch_bounds__(QB) k_integrate(DevSpace S, const double* __restrict__ f,
+                                                  const double* __restrict__ g, int nmom,
+                                                  double* __restrict__ partial) {
+  __shared__ double red[QB];
+  const int nb = gridDim.x, nq = S.nq;
+  const int per = (S.nt + nb - 1) / nb;
+  const int ea = blockIdx.x * per, eb = min(S.nt, ea + per);
+  double acc[MOM];
+  for (int v = 0; v < MOM; ++v) acc[v] = 0.0;
+  for (int e = ea; e < eb; ++e) {
+    const double vol = S.sixv[e] / 6.0;
+    for (int q = threadIdx.x; q < nq; q += QB) {
+      size_t s = (size_t)e * nq + q;
+      double w = vol * S.wq[q] * f[s] * (g ? g[s] : 1.0);
+      acc[0] += w;
+      if (nmom > 1) {
+        double x[3];
+        point(S, e, q, x);
+        acc[1] += w * x[0]; acc[2] += w * x[1]; acc[3] += w * x[2];
+        acc[4] += w * x[0] * x[0]; acc[5] += w * x[0] * x[1]; acc[6] += w * x[0] * x[2];
+        acc[7] += w * x[1] * x[1]; acc[8] += w * x[1] * x[2]; acc[9] += w * x[2] * x[2];
+      }
+    }
+  }
+  for (int v = 0; v < nmom; ++v) {
+    double t = block_reduce(acc[v], red);
+    if (threadIdx.x == 0) partial[(size_t)v * nb + blockIdx.x] = t;
+  }
+}
+
+// out[v] = sum_b partial[v nb + b] in block order
+__global__ void k_sum_partials(const double* __restrict__ partial, int nb, int nv, double* __restrict__ out) {
+  int v = threadIdx.x;
+  if (v >= nv) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += partial[(size_t)v * nb + b];
+  out[v] = s;
+}
+
+void launch_integrate(const DevSpace& S, const double* f, const double* g, int nmom, double* partial, double* out,
+                      hipStream_t st) {
+  int nb = integrate_blocks(S);
+  hipLaunchKernelGGL(k_integrate, dim3(nb), dim3(QB), 0, st, S, f, g, nmom, partial);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partial, nb, nmom, out);
+}
+
+__global__ void __launch_bounds__(QB) k_l2_chunk(DevSpace S, int e0, int e1, const double* __restrict__ xl,
+                                                 const double* __restrict__ u, double* __restrict__ partial) {
+  extern __shared__ double sm[];
+  double* red = sm;        // QB
+  double* xs = sm + QB;    // K
+  const int nb = gridDim.x, nq = S.nq, K = S.K;
+  const int n = e1 - e0, per = (n + nb - 1) / nb;
+  const int ea = e0 + blockIdx.x * per, eb = min(e1, ea + per);
+  double acc = 0.0;
+  for (int e = ea; e < eb; ++e) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < K; i += QB) xs[i] = xl[(size_t)e * K + i];
+    __syncthreads();
+    const double vol = S.sixv[e] / 6.0;
+    for (int q = threadIdx.x; q < nq; q += QB) {
+      double uh = 0.0;
+      for (int i = 0; i < K; ++i) uh += S.phiT[(size_t)i * nq + q] * xs[i];
+      double d = uh - u[(size_t)(e - e0) * nq + q];
+      acc += vol * S.wq[q] * d * d;
+    }
+  }
+  __syncthreads();
+  double t = block_reduce(acc, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+void launch_l2_chunk(const DevSpace& S, int e0, int e1, const double* xl, const double* u, double* partial,
+                     double* out, hipStream_t st) {
+  int n = e1 - e0;
+  if (n <= 0) return;
+  int nb = n < IB ? n : IB;
+  hipLaunchKernelGGL(k_l2_chunk, dim3(nb), dim3(QB), (size_t)(QB + S.K) * 8, st, S, e0, e1, xl, u, partial);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partial, nb, 1, out);
+}
+
+// Perdew & Zunger, Phys. Rev. B 23, 5048 (1981), unpolarized fit to the
+// Ceperley-Alder correlation energy; Slater exchange eps_x = -(3/4)(3 rho/pi)^(1/3).
+__global__ void __launch_bounds__(QB) k_lda_pz(long n, const double* __restrict__ rho, double* __restrict__ vxc,
+                                               double* __restrict__ exc) {
+  long s = (long)blockIdx.x * QB + threadIdx.x;
+  if (s >= n) return;
+  const double r = rho[s];
+  if (!(r > 1e-30)) {
+    if (vxc) vxc[s] = 0.0;
+    if (exc) exc[s] = 0.0;
+    return;
+  }
+  const double PI = 3.14159265358979323846;
+  const double kx = cbrt(3.0 * r / PI);
+  const double ex = -0.75 * kx, vx = -kx;
+  const double rs = cbrt(3.0 / (4.0 * PI * r));
+  double ec, vc;
+  if (rs >= 1.0) {
+    const double g = -0.1423, b1 = 1.0529, b2 = 0.3334;
+    const double sr = sqrt(rs), den = 1.0 + b1 * sr + b2 * rs;
+    ec = g / den;
+    vc = ec * (1.0 + 7.0 / 6.0 * b1 * sr + 4.0 / 3.0 * b2 * rs) / den;
+  } else {
+    const double A = 0.0311, B = -0.048, C = 0.0020, D = -0.0116;
+    const double lr = log(rs);
+    ec = A * lr + B + C * rs * lr + D * rs;
+    vc = A * lr + (B - A / 3.0) + 2.0 / 3.0 * C * rs * lr + (2.0 * D - C) / 3.0 * rs;
+  }
+  if (vxc) vxc[s] = vx + vc;
+  if (exc) exc[s] = r * (ex + ec);
+}
+
+void launch_lda_pz(long n, const double* rho, double* vxc, double* exc, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lda_pz, dim3((unsigned)((n + QB - 1) / QB)), dim3(QB), 0, st, n, rho, vxc, exc);
+}
+
+}  // namespace tsem

@@ -1,0 +1,147 @@
+// C ABI of the quadrature-point work (include/tsem.h): point values, density,
+// integrals and moments, L2 error, LDA exchange-correlation, and the
+// potential that weights the M_V operator.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "handle.hpp"
+
+using namespace tsem;
+
+namespace {
+
+// basis values and weights of the space's 3-D quadrature on the device
+void ensure_quad(tsem_space* sp) {
+  if (sp->qphi.p) return;
+  const Space& S = sp->h;
+  const int K = S.K, nq = sp->nq3;
+  std::vector<double> phi((size_t)nq * K), phiT((size_t)K * nq);
+  std::vector<long double> val(K);
+  for (int q = 0; q < nq; ++q) {
+    S.ref.eval(sp->p3[3 * q], sp->p3[3 * q + 1], sp->p3[3 * q + 2], val.data(), nullptr);
+    for (int i = 0; i < K; ++i) {
+      phi[(size_t)q * K + i] = (double)val[i];
+      phiT[(size_t)i * nq + q] = (double)val[i];
+    }
+  }
+  hipStream_t st = sp->st();
+  upload(sp->qphi, phi, st);
+  upload(sp->qphiT, phiT, st);
+  upload(sp->qw, sp->w3, st);
+  DevSpace& d = sp->d;
+  d.nq = nq;
+  d.phi = sp->qphi.as<double>();
+  d.phiT = sp->qphiT.as<double>();
+  d.wq = sp->qw.as<double>();
+  d.xe = sp->xe.as<double>();
+  d.pts = sp->pts3.as<double>();
+  HIPCHK(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+extern "C" {
+
+int tsem_qp_values(tsem_space* sp, const double* x, int e0, int e1, double* out) {
+  API_BEGIN
+  if (!sp || !x || !out) throw std::invalid_argument("null argument");
+  if (e0 < 0 || e1 > sp->d.nt || e0 > e1) throw std::invalid_argument("element range out of bounds");
+  ensure_quad(sp);
+  sp->qxl.alloc((size_t)sp->d.nt * sp->d.K * 8);
+  launch_gather(sp->d, x, sp->qxl.as<double>(), 0, sp->st());
+  launch_qp_values(sp->d, e0, e1, sp->qxl.as<double>(), out, sp->st());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
+int tsem_density(tsem_space* sp, int nk, const double* psi, const double* occ, double* rho) {
+  API_BEGIN
+  if (!sp || !psi || !occ || !rho) throw std::invalid_argument("null argument");
+  if (nk < 1 || (size_t)nk * sp->d.K * 8 > 64 * 1024) throw std::invalid_argument("number of states out of range");
+  ensure_quad(sp);
+  const DevSpace& d = sp->d;
+  sp->qxl.alloc((size_t)nk * d.nt * d.K * 8);
+  for (int k = 0; k < nk; ++k)
+    launch_gather(d, psi + (size_t)k * d.ndof, sp->qxl.as<double>() + (size_t)k * d.nt * d.K, 0, sp->st());
+  DevBuf docc;
+  upload(docc, std::vector<double>(occ, occ + nk), sp->st());
+  launch_density(d, nk, sp->qxl.as<double>(), docc.as<double>(), rho, sp->st());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
+int tsem_integrate(tsem_space* sp, const double* f, const double* g, int nmom, double* out) {
+  API_BEGIN
+  if (!sp || !f || !out) throw std::invalid_argument("null argument");
+  if (nmom != 1 && nmom != 10) throw std::invalid_argument("nmom must be 1 or 10");
+  ensure_quad(sp);
+  sp->qpart.alloc((size_t)10 * integrate_blocks(sp->d) * 8);
+  sp->qout.alloc(10 * 8);
+  launch_integrate(sp->d, f, g, nmom, sp->qpart.as<double>(), sp->qout.as<double>(), sp->st());
+  HIPCHK(hipMemcpyAsync(out, sp->qout.p, (size_t)nmom * 8, hipMemcpyDeviceToHost, sp->st()));
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
+int tsem_l2_error_chunk(tsem_space* sp, const double* x, int e0, int e1, const double* u, double* sum) {
+  API_BEGIN
+  if (!sp || !x || !u || !sum) throw std::invalid_argument("null argument");
+  if (e0 < 0 || e1 > sp->d.nt || e0 > e1) throw std::invalid_argument("element range out of bounds");
+  ensure_quad(sp);
+  sp->qxl.alloc((size_t)sp->d.nt * sp->d.K * 8);
+  sp->qpart.alloc((size_t)10 * integrate_blocks(sp->d) * 8);
+  sp->qout.alloc(10 * 8);
+  launch_gather(sp->d, x, sp->qxl.as<double>(), 0, sp->st());
+  double h = 0.0;
+  if (e1 > e0) {
+    launch_l2_chunk(sp->d, e0, e1, sp->qxl.as<double>(), u, sp->qpart.as<double>(), sp->qout.as<double>(), sp->st());
+    HIPCHK(hipMemcpyAsync(&h, sp->qout.p, 8, hipMemcpyDeviceToHost, sp->st()));
+  }
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  *sum = h;
+  API_END
+}
+
+int tsem_lda_pz(tsem_space* sp, int64_t n, const double* rho, double* vxc, double* exc) {
+  API_BEGIN
+  if (!sp || !rho) throw std::invalid_argument("null argument");
+  launch_lda_pz((long)n, rho, vxc, exc, sp->st());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
+int tsem_set_potential(tsem_space* sp, const double* v) {
+  API_BEGIN
+  if (!sp) throw std::invalid_argument("null space");
+  ensure_quad(sp);
+  if (!v) {
+    sp->d.vpot = nullptr;
+  } else {
+    size_t bytes = (size_t)sp->d.nt * sp->d.nq * 8;
+    sp->vpot.alloc(bytes);
+    HIPCHK(hipMemcpyAsync(sp->vpot.p, v, bytes, hipMemcpyDeviceToDevice, sp->st()));
+    sp->d.vpot = sp->vpot.as<double>();
+  }
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
+int tsem_apply_potential(tsem_space* sp, double stiff, double mass, double potential, int constrained,
+                         const double* x, double* y) {
+  API_BEGIN
+  if (!sp || !x || !y) throw std::invalid_argument("null argument");
+  if (potential != 0.0 && !sp->d.vpot) throw std::logic_error("no potential set");
+  OpCoef op;
+  op.stiff = stiff; op.mass = mass; op.pot = potential; op.constrained = constrained;
+  launch_apply<double>(sp->d, op, x, y, sp->yl64.as<double>(), nullptr, sp->st());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
+}  // extern "C"

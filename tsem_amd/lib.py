@@ -31,7 +31,7 @@ class PcgStats(C.Structure):
 
 class LobpcgOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("criterion", C.c_int), ("max_iter", C.c_int), ("precond", C.c_int),
-                ("smooth_steps", C.c_int), ("stiff", C.c_double), ("mass", C.c_double)]
+                ("smooth_steps", C.c_int), ("stiff", C.c_double), ("mass", C.c_double), ("potential", C.c_double)]
 
 
 class LobpcgStats(C.Structure):
@@ -74,6 +74,13 @@ SIGNATURES = {
     "tsem_pcg": [_vp, _vp, _vp, C.POINTER(PcgOpts), C.POINTER(PcgStats)],
     "tsem_lobpcg": [_vp, C.c_int, _vp, _dp, C.POINTER(LobpcgOpts), C.POINTER(LobpcgStats)],
     "tsem_inverse_power": [_vp, _vp, C.c_double, C.c_int, C.POINTER(PcgOpts), _dp, C.POINTER(C.c_int)],
+    "tsem_qp_values": [_vp, _vp, C.c_int, C.c_int, _vp],
+    "tsem_density": [_vp, C.c_int, _vp, _dp, _vp],
+    "tsem_integrate": [_vp, _vp, _vp, C.c_int, _dp],
+    "tsem_l2_error_chunk": [_vp, _vp, C.c_int, C.c_int, _vp, _dp],
+    "tsem_lda_pz": [_vp, C.c_int64, _vp, _vp, _vp],
+    "tsem_set_potential": [_vp, _vp],
+    "tsem_apply_potential": [_vp, C.c_double, C.c_double, C.c_double, C.c_int, _vp, _vp],
 }
 
 _lib = None

@@ -77,13 +77,13 @@ def kuhn_cube(n, jitter=0.0, seed=0, lo=0.0, hi=1.0):
 
 
 def graded_box(n, half=10.0, beta=2.0):
-    """[-half, half]^3 Kuhn mesh graded toward the centre by a tanh map."""
-    xyz, tet = kuhn_cube(n, lo=-1.0, hi=1.0)
-    # s in [-1,1] -> half * s-weighted: x = half*(s - tanh(beta*s)/tanh(beta)*0 ...)
-    # points cluster near 0: x = half * sinh(beta*s)/sinh(beta) is graded away from 0;
-    # use the inverse: x = half * atanh(s*tanh(beta))/beta
-    s = xyz
+    """[-half, half]^3 Kuhn mesh graded toward the centre: the uniform grid
+    s in [-1, 1] maps to x = half atanh(s tanh(beta)) / beta, whose spacing
+    is smallest at 0 (by the factor tanh(beta)/beta against the uniform one)
+    and grows toward the faces; the faces stay at exactly +-half."""
+    s, tet = kuhn_cube(n, lo=-1.0, hi=1.0)
     x = half * np.arctanh(s * np.tanh(beta)) / beta
+    x[np.abs(s) == 1.0] = np.sign(s[np.abs(s) == 1.0]) * half
     return x, tet
 
 
