@@ -55,6 +55,13 @@ int tsem_space_destroy(tsem_space* sp);
 /* ctx may be NULL: a host-only space (setup, maps and reference data only;
  * device entry points then return TSEM_ESTATE). */
 
+/* Replace the vertex coordinates of a space (same connectivity, nv x 3
+ * host): volumes and geometric factors are recomputed, the assembled
+ * diagonal is rebuilt on the device, and boundary eliminations and the
+ * potential are cleared (marks stay).  Lets one space serve many jittered
+ * meshes of one topology.  No counterpart (the reference rebuilds TSEM). */
+int tsem_space_set_geometry(tsem_space* sp, const double* xyz);
+
 /* Assembled diagonal of the stiffness operator, host copy (n_dof). */
 int tsem_space_diagonal_host(const tsem_space* sp, double* diag);
 

@@ -67,6 +67,13 @@ class Space:
         self.qorder = qorder
         self.marks = None
 
+    def set_geometry(self, xyz):
+        """New vertex coordinates for the same connectivity (tsem_space_set_geometry);
+        boundary eliminations and the potential are cleared."""
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        _ready()
+        L.check(L.load().tsem_space_set_geometry(self.h, L.dptr(self.xyz)))
+
     def close(self):
         if getattr(self, "h", None):
             try:

@@ -68,7 +68,7 @@ static std::string face_table_path() {
 // bits the tiled gather reads (vertex k: bit k, edge k: bit 4+k, face k: bit
 // 10+k, interior: bit 14).  Eliminations act on whole geometries, so the
 // first DOF of a geometry speaks for all of them; checked here.
-static void upload_mask(tsem_space* sp, hipStream_t st) {
+void upload_mask(tsem_space* sp, hipStream_t st) {
   const Space& S = sp->h;
   const auto& m = sp->cmask;
   auto geo = [&](int64_t start, int n) -> int {
@@ -162,9 +162,9 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
         sp->p3[3 * k + 2] = (double)t[l];
         sp->w3[k] = (double)(6 * (1 - t[j]) * (1 - t[l]) * (1 - t[l]) * w[i] * w[j] * w[l]);
       }
-  S.diagonal(1.0, 0.0, sp->diag);
   sp->cmask.assign(S.ndof, 0);
   if (!ctx) {  // host-only space: setup and map export without a device
+    S.diagonal(1.0, 0.0, sp->diag);
     *out = sp.release();
     return TSEM_OK;
   }
@@ -243,10 +243,7 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   for (size_t i = 0; i < scp.size(); ++i) scp[i] = (int32_t)S.sc_ptr[i];
   upload(sp->sc_ptr, scp, st); upload(sp->sc_code, S.sc_code, st); upload(sp->sc_w, S.sc_w, st);
   upload(sp->sc_w32, std::vector<float>(S.sc_w.begin(), S.sc_w.end()), st);
-  std::vector<double> inv(sp->diag.size());
-  for (size_t i = 0; i < inv.size(); ++i) inv[i] = 1.0 / sp->diag[i];
-  upload(sp->diag_d, sp->diag, st); upload(sp->invdiag, inv, st);
-  upload(sp->invdiag32, std::vector<float>(inv.begin(), inv.end()), st);
+  sp->cmask.assign(S.ndof, 0);
   upload_mask(sp.get(), st);
   std::vector<double> xe((size_t)nt * 12);
   for (int e = 0; e < nt; ++e)
@@ -273,6 +270,23 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   d.diag = sp->diag_d.as<double>(); d.invdiag = sp->invdiag.as<double>(); d.invdiag32 = sp->invdiag32.as<float>();
   d.Dp64 = sp->Dp64.as<double>(); d.Dp32 = sp->Dp32.as<float>();
   d.xt64 = sp->xt64.as<double>(); d.xt32 = sp->xt32.as<float>();
+  // assembled diagonal on the device from the scatter map, host copy for
+  // the boundary-condition setup
+  sp->diag_d.alloc((size_t)S.ndof * 8);
+  sp->invdiag.alloc((size_t)S.ndof * 8);
+  sp->invdiag32.alloc((size_t)S.ndof * 4);
+  d.diag = sp->diag_d.as<double>(); d.invdiag = sp->invdiag.as<double>(); d.invdiag32 = sp->invdiag32.as<float>();
+  {
+    const size_t KK = (size_t)S.K * S.K;
+    std::vector<double> g2(9 * KK);
+    for (int ab = 0; ab < 9; ++ab) std::copy(S.ref.grad2[ab].begin(), S.ref.grad2[ab].end(), g2.begin() + ab * KK);
+    upload(sp->grad2, g2, st);
+  }
+  launch_diagonal(d, sp->grad2.as<double>(), sp->diag_d.as<double>(), sp->invdiag.as<double>(),
+                  sp->invdiag32.as<float>(), st);
+  HIPCHK(hipGetLastError());
+  sp->diag.resize(S.ndof);
+  HIPCHK(hipMemcpyAsync(sp->diag.data(), sp->diag_d.p, (size_t)S.ndof * 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   ctx->nspaces++;
   *out = sp.release();

@@ -80,7 +80,7 @@ struct tsem_space {
   DevBuf sc_ptr, sc_code, sc_w, sc_w32, diag_d, invdiag, invdiag32, cmask_d, xe, pts3, wphi, Dp64, Dp32;
   DevBuf yl64, yl32, bl, partial, scal, xt64, xt32, gmask;
   // quadrature-point work: basis at the points, weights, potential, scratch
-  DevBuf qphi, qphiT, qw, vpot, qxl, qpart, qout;
+  DevBuf qphi, qphiT, qw, vpot, qxl, qpart, qout, grad2;
   // solver vectors
   DevBuf v64[6], v32[6];
   std::unique_ptr<Pmg> pmg;
@@ -90,6 +90,9 @@ struct tsem_space {
     return ctx->stream;
   }
 };
+
+// upload the elimination mask and the per-element geometry bits derived from it
+void upload_mask(tsem_space* sp, hipStream_t st);
 
 #define API_BEGIN try {
 #define API_END                                         \

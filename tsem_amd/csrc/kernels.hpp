@@ -173,4 +173,7 @@ void launch_l2_chunk(const DevSpace& S, int e0, int e1, const double* xl, const 
 // v_xc and energy density e_xc = rho (eps_x + eps_c) per point
 void launch_lda_pz(long n, const double* rho, double* vxc, double* exc, hipStream_t st);
 
+// assembled stiffness diagonal and its inverses from the scatter map (grad2: 9 x K x K)
+void launch_diagonal(const DevSpace& S, const double* grad2, double* diag, double* inv, float* inv32, hipStream_t st);
+
 }  // namespace tsem

@@ -202,4 +202,48 @@ void launch_lda_pz(long n, const double* rho, double* vxc, double* exc, hipStrea
   hipLaunchKernelGGL(k_lda_pz, dim3((unsigned)((n + QB - 1) / QB)), dim3(QB), 0, st, n, rho, vxc, exc);
 }
 
+// Assembled diagonal of the stiffness operator from the transposed gather
+// map: row g's terms come grouped by element, and element e contributes
+// sum_{u,v} w_u w_v K_e[i_u][i_v] over its terms, K_e = sum_ab Ghat_ab(e)
+// grad2[ab] (grad2: 9 x K x K, the reference gradient Gram matrices).
+// Also the inverse diagonal in FP64 and FP32.
+__global__ void __launch_bounds__(QB) k_diagonal(DevSpace S, const double* __restrict__ grad2,
+                                                 double* __restrict__ diag, double* __restrict__ inv,
+                                                 float* __restrict__ inv32) {
+  const int g = blockIdx.x * QB + threadIdx.x;
+  if (g >= S.ndof) return;
+  const int K = S.K;
+  const size_t KK = (size_t)K * K;
+  const int t0 = S.sc_ptr[g], t1 = S.sc_ptr[g + 1];
+  double acc = 0.0;
+  for (int a = t0; a < t1;) {
+    const int e = (S.sc_code[a] & 0x3fffffff) / K;
+    int b = a + 1;
+    while (b < t1 && (S.sc_code[b] & 0x3fffffff) / K == e) ++b;
+    const double* gh = S.ghat + 6 * (size_t)e;
+    const double G[9] = {gh[0], gh[3], gh[4], gh[3], gh[1], gh[5], gh[4], gh[5], gh[2]};
+    double s = 0.0;
+    for (int u = a; u < b; ++u) {
+      int cu = S.sc_code[u], iu = (cu & 0x3fffffff) % K;
+      double wu = cu < 0 ? S.sc_w[u] : ((cu & 0x40000000) ? -1.0 : 1.0);
+      for (int v = a; v < b; ++v) {
+        int cv = S.sc_code[v], iv = (cv & 0x3fffffff) % K;
+        double wv = cv < 0 ? S.sc_w[v] : ((cv & 0x40000000) ? -1.0 : 1.0);
+        double k = 0.0;
+        for (int ab = 0; ab < 9; ++ab) k += G[ab] * grad2[ab * KK + (size_t)iu * K + iv];
+        s += wu * wv * k;
+      }
+    }
+    acc += s;
+    a = b;
+  }
+  diag[g] = acc;
+  inv[g] = 1.0 / acc;
+  inv32[g] = (float)(1.0 / acc);
+}
+
+void launch_diagonal(const DevSpace& S, const double* grad2, double* diag, double* inv, float* inv32, hipStream_t st) {
+  hipLaunchKernelGGL(k_diagonal, dim3((S.ndof + QB - 1) / QB), dim3(QB), 0, st, S, grad2, diag, inv, inv32);
+}
+
 }  // namespace tsem

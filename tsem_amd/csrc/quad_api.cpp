@@ -144,4 +144,40 @@ int tsem_apply_potential(tsem_space* sp, double stiff, double mass, double poten
   API_END
 }
 
+int tsem_space_set_geometry(tsem_space* sp, const double* xyz) {
+  API_BEGIN
+  if (!sp || !xyz) throw std::invalid_argument("null argument");
+  Space& S = sp->h;
+  std::vector<double> keep(S.xyz);
+  S.set_geometry(xyz);
+  for (int t = 0; t < S.topo.nt; ++t)
+    if (!(S.vol[t] > 0)) {
+      S.set_geometry(keep.data());
+      throw std::invalid_argument("degenerate tetrahedron");
+    }
+  hipStream_t st = sp->st();
+  DevSpace& d = sp->d;
+  upload(sp->ghat, S.ghat, st);
+  upload(sp->ghat32, std::vector<float>(S.ghat.begin(), S.ghat.end()), st);
+  upload(sp->sixv, S.sixv, st);
+  upload(sp->sixv32, std::vector<float>(S.sixv.begin(), S.sixv.end()), st);
+  std::vector<double> xe((size_t)S.topo.nt * 12);
+  for (int e = 0; e < S.topo.nt; ++e)
+    for (int k = 0; k < 4; ++k)
+      for (int c = 0; c < 3; ++c) xe[(size_t)e * 12 + 3 * k + c] = S.xyz[3 * (size_t)S.tet[4 * e + k] + c];
+  upload(sp->xe, xe, st);
+  launch_diagonal(d, sp->grad2.as<double>(), sp->diag_d.as<double>(), sp->invdiag.as<double>(),
+                  sp->invdiag32.as<float>(), st);
+  HIPCHK(hipGetLastError());
+  sp->diag.resize(S.ndof);
+  HIPCHK(hipMemcpyAsync(sp->diag.data(), sp->diag_d.p, (size_t)S.ndof * 8, hipMemcpyDeviceToHost, st));
+  // a new geometry starts without boundary conditions and potential
+  std::fill(sp->cmask.begin(), sp->cmask.end(), (uint8_t)0);
+  upload_mask(sp, st);
+  d.vpot = nullptr;
+  sp->pmg.reset();
+  HIPCHK(hipStreamSynchronize(st));
+  API_END
+}
+
 }  // extern "C"

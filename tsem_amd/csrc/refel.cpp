@@ -1,6 +1,10 @@
 // Reference element construction (see refel.hpp).
 #include "refel.hpp"
 
+#include <map>
+#include <memory>
+#include <mutex>
+
 #include <cmath>
 #include <stdexcept>
 
@@ -269,6 +273,18 @@ void RefElement::build(int M_) {
     dfac[a].assign((size_t)Kd * K, 0);
     for (size_t s = 0; s < (size_t)Kd * K; ++s) dfac[a][s] = (double)D[s];
   }
+}
+
+const RefElement& RefElement::cached(int M) {
+  static std::mutex mu;
+  static std::map<int, std::unique_ptr<RefElement>> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto& slot = cache[M];
+  if (!slot) {
+    slot.reset(new RefElement());
+    slot->build(M);
+  }
+  return *slot;
 }
 
 }  // namespace tsem

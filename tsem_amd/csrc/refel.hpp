@@ -44,6 +44,8 @@ struct RefElement {
   // mass factor: mfac[m][i], mfac^T mfac = mass
   std::vector<double> mfac;
   void build(int M_);
+  // the reference element of order M, built once per process (thread safe)
+  static const RefElement& cached(int M);
   // actual basis values (and gradients) at a reference point
   void eval(long double x, long double y, long double z, long double* val, long double* grad /*K*3 or null*/) const;
 };

@@ -110,10 +110,64 @@ int64_t Space::geo_dof(int dim, int i) const {
   return dof_start[off + i];
 }
 
+// Volumes and geometric factors of every element from vertex coordinates
+// (the mesh connectivity stays): pair weights 2/(3V) x dihedral factor
+// (src/Discretize.cpp:1352-1370), the metric Ghat and 6V.
+void Space::set_geometry(const double* xyz_new) {
+  const int nt = topo.nt;
+  if (xyz_new != xyz.data()) xyz.assign(xyz_new, xyz_new + xyz.size());
+  vol.resize(nt);
+  for (int t = 0; t < nt; ++t) {
+    const double* p0 = &xyz[3 * (size_t)tet[4 * t]];
+    double a[3][3];
+    for (int r = 1; r < 4; ++r)
+      for (int d = 0; d < 3; ++d) a[r - 1][d] = xyz[3 * (size_t)tet[4 * t + r] + d] - p0[d];
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                 a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    vol[t] = std::fabs(det) / 6.0;
+  }
+  pairw.assign((size_t)nt * 6, 0.0);
+  ghat.assign((size_t)nt * 6, 0.0);
+  sixv.assign(nt, 0.0);
+  for (int t = 0; t < nt; ++t) {
+    double* w = &pairw[6 * (size_t)t];
+    double coef = 2.0 / (3 * vol[t]);
+    for (int i1 = 0; i1 <= 2; ++i1)
+      for (int i2 = i1 + 1; i2 <= 3; ++i2) {
+        int ip1 = (i1 == 0) ? (i2 % 2 + 1) : 0;
+        int ip2 = 6 - i1 - i2 - ip1;
+        const double* p1 = &xyz[3 * (size_t)tet[4 * t + i1]];
+        const double* p2 = &xyz[3 * (size_t)tet[4 * t + i2]];
+        const double* ps = &xyz[3 * (size_t)tet[4 * t + ip1]];
+        const double* pe = &xyz[3 * (size_t)tet[4 * t + ip2]];
+        double a[3], b[3], c[3];
+        for (int d = 0; d < 3; ++d) { a[d] = pe[d] - ps[d]; b[d] = p1[d] - ps[d]; c[d] = p2[d] - ps[d]; }
+        double u[3] = {a[1] * b[2] - a[2] * b[1], -a[0] * b[2] + a[2] * b[0], a[0] * b[1] - a[1] * b[0]};
+        double vv[3] = {a[1] * c[2] - a[2] * c[1], -a[0] * c[2] + a[2] * c[0], a[0] * c[1] - a[1] * c[0]};
+        double dih = 0.25 * (u[0] * vv[0] + u[1] * vv[1] + u[2] * vv[2]);
+        if (std::fabs(dih) < 1e-12) continue;
+        // pair (0,j): d_j ; pair (i,j): d_j - d_i ; slots as the reference's
+        // multi-index numbering minus 4: (200)0 (110)1 (101)2 (020)3 (011)4 (002)5
+        int slot;
+        if (i1 == 0) slot = (i2 == 1) ? 0 : (i2 == 2) ? 3 : 5;
+        else slot = (i1 == 1 && i2 == 2) ? 1 : (i1 == 1 && i2 == 3) ? 2 : 4;
+        w[slot] += coef * dih;
+      }
+    double* g = &ghat[6 * (size_t)t];
+    g[0] = w[0] + w[1] + w[2];
+    g[1] = w[1] + w[3] + w[4];
+    g[2] = w[2] + w[4] + w[5];
+    g[3] = -w[1];
+    g[4] = -w[2];
+    g[5] = -w[4];
+    sixv[t] = 6 * vol[t];
+  }
+}
+
 void Space::build(int M_, int nv, const double* xyz_, int nt, const int32_t* tet_, const int32_t* femdof_w,
                   const std::string& table_path) {
   M = M_;
-  ref.build(M);
+  ref = RefElement::cached(M);
   K = ref.K;
   table.load(table_path);
   if (table.mmax < M) throw std::runtime_error("face table does not cover this order");
@@ -180,17 +234,6 @@ void Space::build(int M_, int nv, const double* xyz_, int nt, const int32_t* tet
       }
     }
   }
-  vol.resize(nt);
-  for (int t = 0; t < nt; ++t) {
-    const double* p0 = &xyz[3 * (size_t)tet[4 * t]];
-    double a[3][3];
-    for (int r = 1; r < 4; ++r)
-      for (int d = 0; d < 3; ++d) a[r - 1][d] = xyz[3 * (size_t)tet[4 * t + r] + d] - p0[d];
-    double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
-                 a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
-    vol[t] = std::fabs(det) / 6.0;
-  }
-
   // ---- global DOF layout: stable order of all geometries by exact keys ----
   size_t ng = (size_t)nv + ne + nf + nt;
   std::vector<int64_t> key(ng);
@@ -282,43 +325,7 @@ void Space::build(int M_, int nv, const double* xyz_, int nt, const int32_t* tet
       sum_dof[geo_dof(3, i) + d] = a[0] + a[1] + a[2] + 4;
     }
 
-  // ---- geometric factors (src/Discretize.cpp:1352-1370) ----
-  pairw.assign((size_t)nt * 6, 0.0);
-  ghat.assign((size_t)nt * 6, 0.0);
-  sixv.assign(nt, 0.0);
-  for (int t = 0; t < nt; ++t) {
-    double* w = &pairw[6 * (size_t)t];
-    double coef = 2.0 / (3 * vol[t]);
-    for (int i1 = 0; i1 <= 2; ++i1)
-      for (int i2 = i1 + 1; i2 <= 3; ++i2) {
-        int ip1 = (i1 == 0) ? (i2 % 2 + 1) : 0;
-        int ip2 = 6 - i1 - i2 - ip1;
-        const double* p1 = &xyz[3 * (size_t)tet[4 * t + i1]];
-        const double* p2 = &xyz[3 * (size_t)tet[4 * t + i2]];
-        const double* ps = &xyz[3 * (size_t)tet[4 * t + ip1]];
-        const double* pe = &xyz[3 * (size_t)tet[4 * t + ip2]];
-        double a[3], b[3], c[3];
-        for (int d = 0; d < 3; ++d) { a[d] = pe[d] - ps[d]; b[d] = p1[d] - ps[d]; c[d] = p2[d] - ps[d]; }
-        double u[3] = {a[1] * b[2] - a[2] * b[1], -a[0] * b[2] + a[2] * b[0], a[0] * b[1] - a[1] * b[0]};
-        double vv[3] = {a[1] * c[2] - a[2] * c[1], -a[0] * c[2] + a[2] * c[0], a[0] * c[1] - a[1] * c[0]};
-        double dih = 0.25 * (u[0] * vv[0] + u[1] * vv[1] + u[2] * vv[2]);
-        if (std::fabs(dih) < 1e-12) continue;
-        // pair (0,j): d_j ; pair (i,j): d_j - d_i ; slots as the reference's
-        // multi-index numbering minus 4: (200)0 (110)1 (101)2 (020)3 (011)4 (002)5
-        int slot;
-        if (i1 == 0) slot = (i2 == 1) ? 0 : (i2 == 2) ? 3 : 5;
-        else slot = (i1 == 1 && i2 == 2) ? 1 : (i1 == 1 && i2 == 3) ? 2 : 4;
-        w[slot] += coef * dih;
-      }
-    double* g = &ghat[6 * (size_t)t];
-    g[0] = w[0] + w[1] + w[2];
-    g[1] = w[1] + w[3] + w[4];
-    g[2] = w[2] + w[4] + w[5];
-    g[3] = -w[1];
-    g[4] = -w[2];
-    g[5] = -w[4];
-    sixv[t] = 6 * vol[t];
-  }
+  set_geometry(xyz.data());
 
   // ---- transposed gather map for the deterministic scatter ----
   std::vector<std::pair<int64_t, double>> terms;

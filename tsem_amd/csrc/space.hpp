@@ -86,6 +86,7 @@ struct Space {
   void build(int M, int nv, const double* xyz, int nt, const int32_t* tet, const int32_t* femdof_w,
              const std::string& table_path);
   int64_t geo_dof(int dim, int i) const;
+  void set_geometry(const double* xyz);
   // ragged gather map of element e, local mode i: appends (global, weight)
   void gather_terms(int e, int i, std::vector<std::pair<int64_t, double>>& out) const;
   // assembled operator diagonal of  stiff * S + mass_coef * M

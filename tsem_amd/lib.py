@@ -75,6 +75,7 @@ SIGNATURES = {
     "tsem_lobpcg": [_vp, C.c_int, _vp, _dp, C.POINTER(LobpcgOpts), C.POINTER(LobpcgStats)],
     "tsem_inverse_power": [_vp, _vp, C.c_double, C.c_int, C.POINTER(PcgOpts), _dp, C.POINTER(C.c_int)],
     "tsem_qp_values": [_vp, _vp, C.c_int, C.c_int, _vp],
+    "tsem_space_set_geometry": [_vp, _dp],
     "tsem_density": [_vp, C.c_int, _vp, _dp, _vp],
     "tsem_integrate": [_vp, _vp, _vp, C.c_int, _dp],
     "tsem_l2_error_chunk": [_vp, _vp, C.c_int, C.c_int, _vp, _dp],
