@@ -58,6 +58,82 @@ def profile_traffic(kernel, n, p):
     return None
 
 
+def eigen_section(ctx, log, n=16, p=6, m=32, seed=7, tol=1e-8):
+    """Dirichlet Laplacian on the unit cube: the m lowest eigenpairs by block
+    LOBPCG (FP32 p-multigrid preconditioning, FP64 Gram and Rayleigh-Ritz),
+    against the analytic values pi^2 (a^2 + b^2 + c^2)."""
+    import torch
+    from tsem_amd import api, mesh
+    xyz, tet = mesh.kuhn_cube(n)
+    sp = api.Space(ctx, p, xyz, tet, qorder=p + 3)
+    sp.set_cube_marks(0b111111, 0)
+    sp.zero_bc(1)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    lam, X, st = sp.lobpcg(m, seed=seed, tol=tol, precond=4, max_iter=500)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    ab = sorted(a * a + b * b + c * c for a in range(1, 8) for b in range(1, 8) for c in range(1, 8))[:m]
+    exact = np.pi ** 2 * np.array(ab, dtype=np.float64)
+    out = dict(workload="laplace_dirichlet_kuhn%d_p%d_m%d" % (n, p, m), dofs=sp.ndof, seconds=dt,
+               iterations=st["iterations"], applies=st["applies"], max_resid=st["max_resid"],
+               converged=st["converged"], max_rel_err_vs_analytic=float(np.max(np.abs(lam - exact) / exact)),
+               lambda_over_pi2=[float(v) for v in lam / np.pi ** 2])
+    log("eigen %s" % {k: v for k, v in out.items() if k != "lambda_over_pi2"})
+    sp.close()
+    return out
+
+
+def ks_section(ctx, log, n=24, p=5, steps=20, alpha=0.3):
+    """LDA helium (Z = 2, two electrons in the lowest state, 4 states per
+    step) on the tanh-graded [-10, 10]^3 box: 20 SCF steps of linear density
+    mixing."""
+    import torch
+    from tsem_amd.ks import KohnShamAtom
+    t = time.perf_counter()
+    atom = KohnShamAtom(ctx, n=n, p=p, Z=2.0, occ=(2.0, 0.0, 0.0, 0.0))
+    setup = time.perf_counter() - t
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    rec = atom.run(steps=steps, alpha=alpha)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    e = atom.energy()
+    out = dict(workload="ks_helium_graded%d_p%d" % (n, p), dofs=atom.sp.ndof, tets=atom.sp.nt, scf_steps=len(atom.history),
+               seconds=dt, setup_seconds=setup, energy=e, eigenvalues=rec["eigenvalues"],
+               charge=atom.sp.integrate(atom.rho),
+               reference_lda_helium={"energy": -2.8348, "eps_1s": -0.5704})
+    log("kohn-sham %s" % out)
+    atom.sp.close()
+    return out
+
+
+def serve_section(log, count=24, slots=4, seed=2026, load=0.8):
+    """Served Poisson stream: count requests (n in 8..32, p in 3..8, seeded
+    jitter and problem), first all at once (saturated throughput), then with
+    Poisson arrivals at `load` x that rate; each request's latency is set
+    against its solve alone on the device (warm caches)."""
+    from tsem_amd import serve
+    srv = serve.Server(slots=slots)
+    _, sat = srv.run(serve.make_requests(count, seed=seed))
+    rate = load * count / sat["wall_s"]
+    reqs, stream = srv.run(serve.make_requests(count, seed=seed, rate=rate))
+    iso = []
+    for r in serve.make_requests(count, seed=seed):
+        (done,), _ = srv.run([r])
+        iso.append(done.done - done.start)
+    ratio = np.array([(r.done - r.arrival) / i for r, i in zip(reqs, iso)])
+    srv.close()
+    out = dict(workload="served_poisson_n8-32_p3-8", requests=count, slots=slots,
+               saturated={"dofs_per_s": sat["dofs_per_s"], "wall_s": sat["wall_s"],
+                          "requests_per_s": count / sat["wall_s"]},
+               arrival_rate_per_s=rate, stream=stream,
+               latency_over_isolated={"p50": float(np.percentile(ratio, 50)), "p99": float(np.percentile(ratio, 99))},
+               isolated_ms={"p50": float(np.percentile(iso, 50)) * 1e3, "p99": float(np.percentile(iso, 99)) * 1e3})
+    log("serving %s" % out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +145,9 @@ def main():
     ap.add_argument("--precond", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--no-eigen", action="store_true", help="skip the LOBPCG section (16^3, p = 6, 32 pairs)")
+    ap.add_argument("--no-ks", action="store_true", help="skip the Kohn-Sham section (helium, 24^3, p = 5)")
+    ap.add_argument("--serve", type=int, default=24, help="requests of the served-stream section (0 skips it)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="after the solves, copy a buffer of this many MiB three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -199,6 +278,16 @@ def main():
         "kernels": kernels,
         "cpu_baseline": None,
     }
+    for key, skip, fn in (("eigen", args.no_eigen, lambda: eigen_section(ctx, log)),
+                          ("kohn_sham", args.no_ks, lambda: ks_section(ctx, log)),
+                          ("serving", args.serve <= 0, lambda: serve_section(log, count=args.serve))):
+        if skip:
+            continue
+        try:
+            out[key] = fn()
+        except Exception as e:  # reported, not fatal for the headline
+            out[key] = {"error": repr(e)}
+            log("%s failed: %r" % (key, e))
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(p=args.p)

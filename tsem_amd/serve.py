@@ -55,15 +55,17 @@ def footprint(n, p):
 
 def make_requests(count, seed=0, n_range=(8, 32), p_range=(3, 8), rate=None):
     """count requests with n, p uniform in the ranges, seed = request id, and
-    Poisson-process arrivals of the given rate (all at 0 when rate is None)."""
-    rng = np.random.default_rng(seed)
+    Poisson-process arrivals of the given rate (all at 0 when rate is None).
+    The sizes depend on `seed` only, not on the rate."""
+    sizes = np.random.default_rng(seed)
+    gaps = np.random.default_rng(seed + 1)
     t = 0.0
     out = []
     for i in range(count):
         if rate:
-            t += rng.exponential(1.0 / rate)
-        out.append(Request(i, int(rng.integers(n_range[0], n_range[1] + 1)),
-                           int(rng.integers(p_range[0], p_range[1] + 1)), i, t))
+            t += gaps.exponential(1.0 / rate)
+        out.append(Request(i, int(sizes.integers(n_range[0], n_range[1] + 1)),
+                           int(sizes.integers(p_range[0], p_range[1] + 1)), i, t))
     return out
 
 
