@@ -6,15 +6,17 @@ Continuous batching: `slots` workers, each with its own context (HIP
 stream), take the next arrived request as soon as they are free and the
 request's HBM footprint fits the budget, so requests that arrive while
 others are on the GPU start at once and their kernels overlap on the device.
-Each worker keeps its spaces cached per (n, p): a new request of a cached
-topology only replaces the vertex coordinates (tsem_space_set_geometry) and
-rebuilds its load vector and boundary data; cached spaces are dropped (least
-recently used first) when the budget needs their memory.
+Spaces are cached per (n, p) in a pool shared by the workers, each with its
+own context (HIP stream): a new request of a cached topology only replaces
+the vertex coordinates (tsem_space_set_geometry) and rebuilds its load
+vector and boundary data; idle cached spaces are dropped (least recently
+used first) when the budget needs their memory.  Workers are split into a
+small-request lane and a general lane, so small requests never queue behind
+large ones.
 
 Latency of a request = completion time - arrival time (queueing included);
 `Server.run` returns per-request records and p50/p99 summaries.
 """
-import heapq
 import threading
 import time
 from dataclasses import dataclass
@@ -39,6 +41,7 @@ class Request:
     iterations: int = 0
     relres: float = None
     worker: int = -1
+    built: bool = False           # the request's space had to be built
     solution: object = None
     error: str = None
 
@@ -51,6 +54,17 @@ def footprint(n, p):
     N = (p * n + 1) ** 3
     terms = 3.2 * N
     return int(terms * 16 + E * K * 24 + E * 200 + N * 150) + (64 << 20)
+
+
+def cost(n, p):
+    """Work estimate of a request: DOFs x order x cells per edge (PCG
+    iterations grow with both)."""
+    return (p * n + 1) ** 3 * p * n
+
+
+def median_cost(n_range=(8, 32), p_range=(3, 8)):
+    return float(np.median([cost(n, p) for n in range(n_range[0], n_range[1] + 1)
+                            for p in range(p_range[0], p_range[1] + 1)]))
 
 
 def make_requests(count, seed=0, n_range=(8, 32), p_range=(3, 8), rate=None):
@@ -70,79 +84,98 @@ def make_requests(count, seed=0, n_range=(8, 32), p_range=(3, 8), rate=None):
 
 
 class Server:
-    def __init__(self, device=0, slots=4, hbm_budget=None, tol=1e-10, keep_solutions=False):
+    def __init__(self, device=0, slots=8, small_slots=None, small_cost=None, hbm_budget=None, tol=1e-10,
+                 keep_solutions=False):
+        """slots workers; the first small_slots of them (default half) take only
+        requests whose cost is at most small_cost (default: the median over the
+        request distribution), so small requests never queue behind large
+        ones; the other workers take the earliest large request, else the
+        earliest small one."""
         import torch
         self.device = device
         self.slots = slots
+        self.small_slots = slots // 2 if small_slots is None else small_slots
+        self.small_cost = median_cost() if small_cost is None else small_cost
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(device)
             hbm_budget = int(0.7 * free)
         self.budget = hbm_budget
         self.tol = tol
         self.keep = keep_solutions
-        self.ctx = [api.Context(device) for _ in range(slots)]
-        self.cache = [dict() for _ in range(slots)]   # worker -> {(n, p): (space, last use)}
-        self.busy = [None] * slots                    # (n, p) a worker is solving on
+        # cached spaces per (n, p), shared by all workers; each space has its
+        # own context (stream) and serves one request at a time
+        self.pool = {}                                # (n, p) -> [[space, ctx, busy, last use], ...]
         self.in_use = 0                               # estimated bytes of all spaces held
         self.lock = threading.Condition()
         self.pending = []
         self.t0 = None
 
     # ------------------------------------------------------------- memory --
-    def _reserve(self, w, key):
-        """With the lock held: True when worker w may take a request of
-        topology key, evicting idle cached spaces (least recently used
-        first) to fit the budget; an idle device takes any request."""
-        need = 0 if key in self.cache[w] else footprint(*key)
+    def _reserve(self, key):
+        """With the lock held: an entry of the pool for topology key, marked
+        busy (an idle cached space, or a placeholder to build when the budget
+        allows after evicting idle spaces, least recently used first); None
+        when the request must wait.  An idle device takes any request."""
+        for ent in self.pool.get(key, []):
+            if not ent[2]:
+                ent[2] = True
+                return ent
+        need = footprint(*key)
         while self.in_use + need > self.budget:
-            victims = [(t, ww, k) for ww, c in enumerate(self.cache) for k, (sp, t) in c.items()
-                       if not (ww == w and k == key) and self.busy[ww] != k]
+            victims = [(ent[3], k, i) for k, lst in self.pool.items() for i, ent in enumerate(lst) if not ent[2]]
             if not victims:
-                if any(b is not None for b in self.busy):
-                    return False
+                if any(ent[2] for lst in self.pool.values() for ent in lst):
+                    return None
                 break
-            _, ww, k = min(victims)
-            sp, _ = self.cache[ww].pop(k)
-            if sp is not None:
-                sp.close()
+            _, k, i = min(victims)
+            sp, ctx, _, _ = self.pool[k].pop(i)
+            sp.close()
+            ctx.close()
             self.in_use -= footprint(*k)
         self.in_use += need
-        if need:
-            self.cache[w][key] = (None, time.perf_counter())  # placeholder until built
-        self.busy[w] = key
-        return True
+        ent = [None, None, True, time.perf_counter()]
+        self.pool.setdefault(key, []).append(ent)
+        return ent
 
     # ------------------------------------------------------------ workers --
     def _next(self, w):
+        """The next request for worker w (arrival order within its lane) and
+        its pool entry, or None when no request is left."""
+        small_lane = w < self.small_slots
         with self.lock:
             while True:
                 if not self.pending:
                     return None
-                req = self.pending[0][2]
                 now = time.perf_counter() - self.t0
-                if req.arrival > now:
-                    self.lock.wait(timeout=req.arrival - now)
-                    continue
-                if self._reserve(w, (req.n, req.p)):
-                    heapq.heappop(self.pending)
-                    req.start = time.perf_counter() - self.t0
-                    return req
-                self.lock.wait(timeout=0.01)
+                arrived = [r for r in self.pending if r.arrival <= now]
+                small = [r for r in arrived if cost(r.n, r.p) <= self.small_cost]
+                large = [r for r in arrived if cost(r.n, r.p) > self.small_cost]
+                pick = (small[:1] if small_lane else (large[:1] or small[:1]))
+                if pick:
+                    ent = self._reserve((pick[0].n, pick[0].p))
+                    if ent is not None:
+                        req = pick[0]
+                        self.pending.remove(req)
+                        req.start = now
+                        return req, ent
+                # wait for an arrival or a finished request
+                upcoming = [r.arrival - now for r in self.pending if r.arrival > now]
+                self.lock.wait(timeout=min(upcoming + [0.05]))
 
-    def _space(self, w, n, p):
-        key = (n, p)
-        with self.lock:
-            sp, _ = self.cache[w].get(key, (None, 0))
-        if sp is None:
+    def _space(self, ent, n, p, req=None):
+        if ent[0] is None:
+            if req is not None:
+                req.built = True
+            ctx = api.Context(self.device)
             xyz, tet = mesh.kuhn_cube(n)
-            sp = api.Space(self.ctx[w], p, xyz, tet, qorder=p + 3)
-        with self.lock:
-            self.cache[w][key] = (sp, time.perf_counter())
-        return sp
+            sp = api.Space(ctx, p, xyz, tet, qorder=p + 3)
+            with self.lock:
+                ent[0], ent[1] = sp, ctx
+        return ent[0]
 
-    def solve_one(self, w, req):
-        """Setup and solve of one request on worker w's context."""
-        sp = self._space(w, req.n, req.p)
+    def solve_one(self, ent, req):
+        """Setup and solve of one request on the space of a pool entry."""
+        sp = self._space(ent, req.n, req.p, req)
         xyz, _ = mesh.kuhn_cube(req.n, jitter=0.1, seed=req.seed)
         sp.set_geometry(xyz)
         b = api.poisson_setup(sp, problems.Manufactured(req.seed))
@@ -157,25 +190,29 @@ class Server:
         import torch
         torch.cuda.set_device(self.device)
         while True:
-            req = self._next(w)
-            if req is None:
+            nxt = self._next(w)
+            if nxt is None:
                 return
+            req, ent = nxt
             try:
-                self.solve_one(w, req)
+                self.solve_one(ent, req)
             except Exception as e:  # reported by run()
                 req.error = repr(e)
             finally:
                 req.done = time.perf_counter() - self.t0
                 req.worker = w
                 with self.lock:
-                    self.busy[w] = None
+                    if ent[0] is None:  # build failed: give the memory back
+                        self.pool[(req.n, req.p)].remove(ent)
+                        self.in_use -= footprint(req.n, req.p)
+                    ent[2] = False
+                    ent[3] = time.perf_counter()
                     self.lock.notify_all()
 
     def run(self, requests):
         """Serve the requests (arrival times relative to now); returns them
         with timings filled in, plus a summary."""
-        self.pending = [(r.arrival, r.id, r) for r in requests]
-        heapq.heapify(self.pending)
+        self.pending = sorted(requests, key=lambda r: (r.arrival, r.id))
         self.t0 = time.perf_counter()
         threads = [threading.Thread(target=self._worker, args=(w,), daemon=True) for w in range(self.slots)]
         for t in threads:
@@ -193,11 +230,11 @@ class Server:
                               max_relres=max(r.relres for r in requests))
 
     def close(self):
-        for c in self.cache:
-            for sp, _ in c.values():
-                if sp is not None:
-                    sp.close()
-            c.clear()
-        self.in_use = 0
-        for ctx in self.ctx:
-            ctx.close()
+        with self.lock:
+            for lst in self.pool.values():
+                for sp, ctx, _, _ in lst:
+                    if sp is not None:
+                        sp.close()
+                        ctx.close()
+            self.pool.clear()
+            self.in_use = 0
