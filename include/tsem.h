@@ -156,8 +156,10 @@ int tsem_scatter(tsem_space* sp, const double* yl_dev, double* y_dev);
 int tsem_timing(int enable);
 int tsem_timing_read(double* out);
 
-/* Element-kernel selection for the operator (process wide): 0 = MFMA
- * kernel where it applies (default), 1 = scalar kernel.  No counterpart. */
+/* Element-kernel selection for the operator (process wide): 0 = fastest
+ * matrix-core path (default: fp16-split MFMA for FP32 vectors, FP64 MFMA
+ * for FP64), 1 = scalar kernel, 2 = FP32/FP64 MFMA (no fp16 splitting).
+ * No counterpart. */
 int tsem_set_apply_path(int path);
 
 /* Assembled diagonal (device copy out, n_dof). */

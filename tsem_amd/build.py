@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtsem_amd.so")
 SOURCES = ["refel.cpp", "space.cpp", "capi.cpp", "eigen.cpp", "quad_api.cpp", "kernels.hip", "apply_mfma.hip",
-           "pmg.hip", "block.hip", "quad.hip"]
+           "apply_mfma_h.hip", "pmg.hip", "block.hip", "quad.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -154,8 +154,9 @@ __global__ void __launch_bounds__(NT) k_elem_gemm(DevSpace S, OpCoef op, const T
                                                   const T* __restrict__ xt, T* __restrict__ yl) {
   using v4 = typename Mma<T>::v4;
   constexpr int NE = 16 * CT;
+  constexpr int VS = NE + 1;  // odd row stride of the LDS tiles: no bank conflicts
   extern __shared__ unsigned char smem_raw[];
-  T* V = reinterpret_cast<T*>(smem_raw);  // [3 Mp][NE]
+  T* V = reinterpret_cast<T*>(smem_raw);  // [3 Mp][VS]
   const int K = S.K;
   const int Kl = op.Kl > 0 ? op.Kl : K;
   const int Kdl = op.Kdl > 0 ? op.Kdl : S.Kd;
@@ -209,71 +210,87 @@ __global__ void __launch_bounds__(NT) k_elem_gemm(DevSpace S, OpCoef op, const T
         int m = 16 * t + Mma<T>::row(lk, j);
         T u0 = acc[0][c][j], u1 = acc[1][c][j], u2 = acc[2][c][j];
         T keep = (m < Kdl) ? T(1) : T(0);
-        V[(size_t)(0 * Mp + m) * NE + col] = keep * (g0 * u0 + g3 * u1 + g4 * u2);
-        V[(size_t)(1 * Mp + m) * NE + col] = keep * (g3 * u0 + g1 * u1 + g5 * u2);
-        V[(size_t)(2 * Mp + m) * NE + col] = keep * (g4 * u0 + g5 * u1 + g2 * u2);
+        V[(size_t)(0 * Mp + m) * VS + col] = keep * (g0 * u0 + g3 * u1 + g4 * u2);
+        V[(size_t)(1 * Mp + m) * VS + col] = keep * (g3 * u0 + g1 * u1 + g5 * u2);
+        V[(size_t)(2 * Mp + m) * VS + col] = keep * (g4 * u0 + g5 * u1 + g2 * u2);
       }
     }
   }
   __syncthreads();
 
-  // ---- GEMM 2: Y[i][e] = sum_r Dp[r][i] V[r][e].  Wave w owns column tile
-  // w % CT and row tiles w / CT + j (4 / CT), j = 0.. (compile-time slots)
-  constexpr int RSTEP = WAVES / CT;
-  constexpr int RW = (RMAX + RSTEP - 1) / RSTEP;  // row-tile slots per wave
-  const int c = wave % CT, r0 = wave / CT;
-  v4 acc[RW];
+  // ---- GEMM 2: Y[i][e] = sum_r Dp[r][i] V[r][e].  Wave w owns row tiles
+  // w + WAVES j and every column tile; per 16 rows it issues all factor
+  // loads (global) and V loads (LDS) first, then the MFMAs, so each factor
+  // value serves all CT column tiles
+  constexpr int RJ = (RMAX + WAVES - 1) / WAVES;
+  v4 acc[RJ][CT];
 #pragma unroll
-  for (int j = 0; j < RW; ++j) acc[j] = v4{0, 0, 0, 0};
-  int kend[RW];
+  for (int j = 0; j < RJ; ++j)
 #pragma unroll
-  for (int j = 0; j < RW; ++j) {
-    int t2 = r0 + RSTEP * j;
-    kend[j] = (t2 < Rt) ? min(S.kend2[t2], 16 * Mt) : 0;
-  }
+    for (int c = 0; c < CT; ++c) acc[j][c] = v4{0, 0, 0, 0};
+  int kend[RJ];
   int kmax = 0;
 #pragma unroll
-  for (int j = 0; j < RW; ++j) kmax = max(kmax, kend[j]);
+  for (int j = 0; j < RJ; ++j) {
+    int t2 = wave + WAVES * j;
+    kend[j] = (t2 < Rt) ? min(S.kend2[t2], 16 * Mt) : 0;
+    kmax = max(kmax, kend[j]);
+  }
   for (int a = 0; a < 3; ++a) {
-    const T* vb = V + (size_t)(a * Mp) * NE + 16 * c + lr;
-    const T* db = Dp + (size_t)(a * Mp + lk) * Cp + 16 * r0 + lr;
-    for (int k = 0; k < kmax; k += 4) {
-      T b = vb[(size_t)(k + lk) * NE];
+    for (int k0 = 0; k0 < kmax; k0 += 16) {
+      T av[4][RJ], bv[4][CT];
 #pragma unroll
-      for (int j = 0; j < RW; ++j) {
-        if (k < kend[j]) {
-          T av = db[(size_t)k * Cp + 16 * RSTEP * j];
-          acc[j] = Mma<T>::op(av, b, acc[j]);
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + 4 * u;
+#pragma unroll
+        for (int j = 0; j < RJ; ++j) {
+          int t2 = wave + WAVES * j;
+          int i = 16 * (t2 < Rt ? t2 : 0) + lr;
+          av[u][j] = (k < kmax) ? Dp[(size_t)(a * Mp + k + lk) * Cp + i] : T(0);
         }
+#pragma unroll
+        for (int c = 0; c < CT; ++c) bv[u][c] = (k < kmax) ? V[(size_t)(a * Mp + k + lk) * VS + 16 * c + lr] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + 4 * u;
+#pragma unroll
+        for (int j = 0; j < RJ; ++j)
+          if (k < kend[j]) {
+#pragma unroll
+            for (int c = 0; c < CT; ++c) acc[j][c] = Mma<T>::op(av[u][j], bv[u][c], acc[j][c]);
+          }
       }
     }
   }
   // ---- write Y element-major (yl[e][i]) through an LDS transpose
   __syncthreads();  // every wave is done reading V
-  T* Y = V;         // [K][NE]
+  T* Y = V;         // [K][VS]
 #pragma unroll
-  for (int j = 0; j < RW; ++j) {
-    int t2 = r0 + RSTEP * j;
+  for (int j = 0; j < RJ; ++j) {
+    int t2 = wave + WAVES * j;
     if (t2 < Rt) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        int i = 16 * t2 + Mma<T>::row(lk, q);
-        if (i < K) Y[(size_t)i * NE + 16 * c + lr] = acc[j][q];
-      }
+      for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          int i = 16 * t2 + Mma<T>::row(lk, q);
+          if (i < K) Y[(size_t)i * VS + 16 * c + lr] = acc[j][c][q];
+        }
     }
   }
   __syncthreads();
   const int nval = min(NE, S.nt - e0);
   for (int s = tid; s < nval * K; s += NT) {
     int le = s / K, i = s % K;
-    yl[(size_t)(e0 + le) * K + i] = (i < Kl && i < 16 * Rt) ? Y[(size_t)i * NE + le] : T(0);
+    yl[(size_t)(e0 + le) * K + i] = (i < Kl && i < 16 * Rt) ? Y[(size_t)i * VS + le] : T(0);
   }
 }
 
 template <typename T, int CT>
 static size_t gemm_lds(const DevSpace& S) {
   size_t rows = (size_t)3 * S.Mp > (size_t)S.K ? (size_t)3 * S.Mp : (size_t)S.K;
-  return rows * 16 * CT * sizeof(T);
+  return rows * (16 * CT + 1) * sizeof(T);
 }
 
 bool mfma_supported(const DevSpace& S) {

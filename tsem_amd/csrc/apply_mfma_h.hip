@@ -1,0 +1,377 @@
+// FP32 element operator on the fp16 matrix cores with two-term splitting.
+//
+// Every FP32 operand a is carried as a_hi + a_lo, both fp16, after scaling so
+// that |a| <= 1 (per element for the element vectors, one power of two for
+// the reference factors): a_hi = fp16(a), a_lo = fp16(a - a_hi).  A product
+// a b is formed as a_hi b_hi + a_hi b_lo + a_lo b_hi on
+// v_mfma_f32_16x16x32_f16 with FP32 accumulation; the dropped a_lo b_lo and
+// the rounding of the parts leave a relative error of about 2^-22 per
+// product (FP32 itself: 2^-24), at 16x the FP32 MFMA rate per instruction.
+//
+// k_gather_tiles_h: element coefficients of a 32-element tile, scaled by the
+//   element's max |x|, split into hi/lo fp16 groups of 8 consecutive modes:
+//   xh[tile][k / 8][element][8] (the B-operand layout of the MFMA), scale
+//   xs[e] = max |x_e|.
+// k_elem_gemm_h: per block of NE = 16 CT elements,
+//   U = D X (GEMM 1), V = Ghat U per element (FP32, into LDS), exact per-
+//   element max |V| (LDS atomics), V / max split to fp16 hi/lo in LDS
+//   (element-major, the B-operand layout), Y = D^T V (GEMM 2), Y rescaled
+//   and written element-major through an LDS transpose.
+//
+// MFMA 16x16x32 f16 operand layout (one 64-lane wave, measured on gfx950):
+// lane l supplies A[l % 16][8 (l / 16) + t] and B[8 (l / 16) + t][l % 16],
+// t = 0..7; accumulator value j of lane l is C[4 (l / 16) + j][l % 16].
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace tsem {
+
+namespace {
+constexpr int WAVES = 4;
+constexpr int NT = 64 * WAVES;
+constexpr int RMAX = 12;  // row tiles of GEMM 2 (K <= 192)
+constexpr int GPT = 6;    // fp16 groups of V per thread in the split
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f4 mma(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+
+__device__ inline void split8(const float* v, float inv, h8& hi, h8& lo) {
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    float s = v[t] * inv;
+    _Float16 h = (_Float16)s;
+    hi[t] = h;
+    lo[t] = (_Float16)(s - (float)h);
+  }
+}
+
+__device__ inline h8 ld8(const uint16_t* p) { return *reinterpret_cast<const h8*>(p); }
+__device__ inline void st8(uint16_t* p, h8 v) { *reinterpret_cast<h8*>(p) = v; }
+}  // namespace
+
+// one block per 32-element tile
+__global__ void __launch_bounds__(256) k_gather_tiles_h(DevSpace S, int Kl, const float* __restrict__ x,
+                                                        uint16_t* __restrict__ xhi, uint16_t* __restrict__ xlo,
+                                                        float* __restrict__ xs, int constrained) {
+  __shared__ ElemDesc sd[TILE];
+  __shared__ int sm[TILE];
+  __shared__ float smax[TILE];
+  extern __shared__ float vf[];  // [K32][TILE]
+  const int tile = xcd_block(blockIdx.x, gridDim.x);
+  const int e0 = tile * TILE;
+  const int nvalid = min(TILE, S.nt - e0);
+  {
+    const int32_t* src = reinterpret_cast<const int32_t*>(S.desc + e0);
+    int32_t* dst = reinterpret_cast<int32_t*>(sd);
+    constexpr int W = sizeof(ElemDesc) / 4;
+    for (int s = threadIdx.x; s < TILE * W; s += 256) dst[s] = (s / W < nvalid) ? src[s] : 0;
+    for (int s = threadIdx.x; s < TILE; s += 256) sm[s] = (constrained && s < nvalid) ? S.gmask[e0 + s] : 0;
+  }
+  __syncthreads();
+  const int K32 = S.K32;
+  const int K32l = (Kl + 31) & ~31;
+  for (int r = threadIdx.x; r < K32l * TILE; r += 256) {
+    int k = r / TILE, le = r % TILE, e = e0 + le;
+    float v = 0.0f;
+    if (le < nvalid && k < Kl) {
+      const ElemDesc& D = sd[le];
+      const int msk = sm[le];
+      int code = S.slot[k];
+      int kind = code & 3, geo = (code >> 2) & 7, sub = code >> 5;
+      if (kind == 0) {
+        v = ((msk >> geo) & 1) ? 0.0f : x[D.v[geo]];
+      } else if (kind == 1) {
+        int ed = D.e[geo];
+        v = ((msk >> (4 + geo)) & 1) ? 0.0f : x[(ed & 0x7fffffff) + sub];
+        if (ed < 0 && (sub & 1)) v = -v;
+      } else if (kind == 3) {
+        v = ((msk >> 14) & 1) ? 0.0f : x[D.t + sub];
+      } else {
+        int fo = D.fo[geo];
+        int row = (fo & 7) * S.nface + sub;
+        const bool fmask = (msk >> (10 + geo)) & 1;
+        float acc = 0.0f;
+        for (int t = S.ft_ptr[row]; t < S.ft_ptr[row + 1]; ++t) {
+          int key = S.ft_key[t];
+          int ie = key & 3, idx = key >> 2;
+          float w = (float)S.ft_w[t];
+          float xv;
+          if (ie < 3) {
+            int le2 = (fo >> (3 + 3 * ie)) & 7;
+            if (((fo >> (12 + ie)) & 1) && (idx & 1)) w = -w;
+            xv = ((msk >> (4 + le2)) & 1) ? 0.0f : x[(D.e[le2] & 0x7fffffff) + idx];
+          } else {
+            xv = fmask ? 0.0f : x[D.f[geo] + idx];
+          }
+          acc += xv * w;
+        }
+        v = acc;
+      }
+    }
+    vf[k * TILE + le] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < TILE) {
+    float m = 0.0f;
+    for (int k = 0; k < K32l; ++k) m = fmaxf(m, fabsf(vf[k * TILE + threadIdx.x]));
+    smax[threadIdx.x] = m;
+    if (e0 + (int)threadIdx.x < S.nt) xs[e0 + threadIdx.x] = m;
+  }
+  __syncthreads();
+  // groups of 8 modes of one element: xh[tile][kb][le][8]
+  const int KB = K32 / 8, KBl = K32l / 8;
+  for (int g = threadIdx.x; g < KBl * TILE; g += 256) {
+    int kb = g / TILE, le = g % TILE;
+    float v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = vf[(8 * kb + t) * TILE + le];
+    float m = smax[le];
+    h8 hi, lo;
+    split8(v, m > 0.0f ? 1.0f / m : 0.0f, hi, lo);
+    size_t o = (((size_t)tile * KB + kb) * TILE + le) * 8;
+    st8(xhi + o, hi);
+    st8(xlo + o, lo);
+  }
+}
+
+// at least 3 waves per SIMD: the phases are latency bound, occupancy pays
+// more than registers (measured: 162 -> 128 us per launch on the 32^3, p = 6
+// mesh, no spills)
+template <int CT>
+__global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, const uint16_t* __restrict__ xhi,
+                                                    const uint16_t* __restrict__ xlo, const float* __restrict__ xs,
+                                                    float* __restrict__ yl) {
+  constexpr int NE = 16 * CT;
+  constexpr int VS = NE + 1;  // odd row stride of the FP32 LDS tiles: no bank conflicts
+  extern __shared__ unsigned char smem_raw[];
+  float* Vf = reinterpret_cast<float*>(smem_raw);                    // [3 Mp][VS]  (phase A)
+  uint16_t* Vh = reinterpret_cast<uint16_t*>(smem_raw);              // hi [NE][R2], lo [NE][R2]  (phase C)
+  unsigned int* colmax = reinterpret_cast<unsigned int*>(smem_raw + S.hlds_v);  // [NE] float bits
+  const int K = S.K, K32 = S.K32, Mp = S.Mp, R2 = S.R2;
+  const int Kl = op.Kl > 0 ? op.Kl : K;
+  const int Kdl = op.Kdl > 0 ? op.Kdl : S.Kd;
+  const int Mt = (Kdl + 15) >> 4;
+  const int Rt = (Kl + 15) >> 4;
+  const int K32l = (Kl + 31) & ~31;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int e0 = blockIdx.x * NE;
+  const int KB = K32 / 8;
+  const float cst = (float)op.stiff * S.dinv;  // undo the factor scaling of GEMM 1
+  for (int s = tid; s < NE; s += NT) colmax[s] = 0u;
+  __syncthreads();
+
+  // ---- phase A: GEMM 1 + mixing; wave w owns m tiles w, w + WAVES, ...
+  for (int t = wave; t < Mt; t += WAVES) {
+    f4 acc[3][CT];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int c = 0; c < CT; ++c) acc[a][c] = f4{0, 0, 0, 0};
+    size_t xo[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      int e = e0 + 16 * c + lr;
+      xo[c] = ((size_t)(e >> 5) * KB * TILE + (e & 31)) * 8;
+    }
+    const int kb1 = S.kbeg1h[t];
+    for (int kk = kb1; kk < K32l; kk += 32) {
+      h8 ah[3], al[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        size_t o = ((size_t)(a * Mp + 16 * t + lr) * K32 + kk + 8 * lk);
+        ah[a] = ld8(S.Dh1hi + o);
+        al[a] = ld8(S.Dh1lo + o);
+      }
+      const size_t ko = (size_t)(kk / 8 + lk) * TILE * 8;
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        h8 bh = ld8(xhi + xo[c] + ko), bl = ld8(xlo + xo[c] + ko);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          acc[a][c] = mma(ah[a], bh, acc[a][c]);
+          acc[a][c] = mma(ah[a], bl, acc[a][c]);
+          acc[a][c] = mma(al[a], bh, acc[a][c]);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      int col = 16 * c + lr, e = e0 + col;
+      float g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0, g5 = 0;
+      if (e < S.nt) {
+        const float* g = S.ghat32 + 6 * (size_t)e;
+        float f = cst * xs[e];
+        g0 = f * g[0]; g1 = f * g[1]; g2 = f * g[2]; g3 = f * g[3]; g4 = f * g[4]; g5 = f * g[5];
+      }
+      float vmax = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int m = 16 * t + 4 * lk + j;
+        float u0 = acc[0][c][j], u1 = acc[1][c][j], u2 = acc[2][c][j];
+        float keep = (m < Kdl) ? 1.0f : 0.0f;
+        float v0 = keep * (g0 * u0 + g3 * u1 + g4 * u2);
+        float v1 = keep * (g3 * u0 + g1 * u1 + g5 * u2);
+        float v2 = keep * (g4 * u0 + g5 * u1 + g2 * u2);
+        Vf[(size_t)(0 * Mp + m) * VS + col] = v0;
+        Vf[(size_t)(1 * Mp + m) * VS + col] = v1;
+        Vf[(size_t)(2 * Mp + m) * VS + col] = v2;
+        vmax = fmaxf(vmax, fmaxf(fabsf(v0), fmaxf(fabsf(v1), fabsf(v2))));
+      }
+      atomicMax(&colmax[col], __float_as_uint(vmax));
+    }
+  }
+  // rows of V beyond the m tiles computed stay zero for GEMM 2
+  for (int s = tid; s < 3 * Mp * NE; s += NT) {
+    int r = s / NE, m = r % Mp;
+    if (m >= 16 * Mt) Vf[(size_t)r * VS + s % NE] = 0.0f;
+  }
+  __syncthreads();
+
+  // ---- phase B: V / max split to fp16, element-major with each derivative
+  // component padded to Mh (a multiple of 32) rows; in place: read all,
+  // then write
+  const int Mh = S.Mh;
+  const int ngroups = 3 * (Mh / 8) * NE;
+  float vbuf[GPT][8];
+#pragma unroll
+  for (int q = 0; q < GPT; ++q) {
+    int g = tid + q * NT;
+    if (g < ngroups) {
+      int e = g % NE, r8 = g / NE;
+      int a = r8 / (Mh / 8), m0 = 8 * (r8 % (Mh / 8));
+#pragma unroll
+      for (int t = 0; t < 8; ++t) vbuf[q][t] = (m0 + t < Mp) ? Vf[(size_t)(a * Mp + m0 + t) * VS + e] : 0.0f;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < GPT; ++q) {
+    int g = tid + q * NT;
+    if (g < ngroups) {
+      int e = g % NE, r8 = g / NE;
+      float m = __uint_as_float(colmax[e]);
+      h8 hi, lo;
+      split8(vbuf[q], m > 0.0f ? 1.0f / m : 0.0f, hi, lo);
+      st8(Vh + (size_t)e * R2 + 8 * r8, hi);
+      st8(Vh + (size_t)(NE + e) * R2 + 8 * r8, lo);
+    }
+  }
+  __syncthreads();
+
+  // ---- phase C: GEMM 2, Y[i][e] = sum_r D[r][i] V[r][e].  Wave w owns row
+  // tiles w + WAVES j and every column tile: per 32-row step it issues all its
+  // factor loads (global) and the column tiles' V loads (LDS) at once, then
+  // the MFMAs, so each factor load serves all CT column tiles
+  constexpr int RJ = (RMAX + WAVES - 1) / WAVES;
+  f4 acc[RJ][CT];
+#pragma unroll
+  for (int j = 0; j < RJ; ++j)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[j][c] = f4{0, 0, 0, 0};
+  int kend[RJ];
+  int kmax = 0;
+#pragma unroll
+  for (int j = 0; j < RJ; ++j) {
+    int t2 = wave + WAVES * j;
+    kend[j] = (t2 < Rt) ? min(S.kend2h[t2], 16 * Mt) : 0;
+    kmax = max(kmax, kend[j]);
+  }
+  for (int a = 0; a < 3; ++a) {
+    for (int mm = 0; mm < kmax; mm += 32) {
+      const int rr = a * Mh + mm + 8 * lk;
+      h8 ah[RJ], al[RJ], bh[CT], bl[CT];
+#pragma unroll
+      for (int j = 0; j < RJ; ++j) {
+        int t2 = wave + WAVES * j;
+        int i = 16 * (t2 < Rt ? t2 : 0) + lr;
+        size_t o = (size_t)i * S.R2g + rr;
+        ah[j] = ld8(S.Dh2hi + o);
+        al[j] = ld8(S.Dh2lo + o);
+      }
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        int col = 16 * c + lr;
+        bh[c] = ld8(Vh + (size_t)col * R2 + rr);
+        bl[c] = ld8(Vh + (size_t)(NE + col) * R2 + rr);
+      }
+#pragma unroll
+      for (int j = 0; j < RJ; ++j) {
+        if (mm < kend[j]) {
+#pragma unroll
+          for (int c = 0; c < CT; ++c) {
+            acc[j][c] = mma(ah[j], bh[c], acc[j][c]);
+            acc[j][c] = mma(ah[j], bl[c], acc[j][c]);
+            acc[j][c] = mma(al[j], bh[c], acc[j][c]);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();  // every wave is done reading Vh
+
+  // ---- phase D: rescale and write Y element-major through an LDS transpose
+  float* Y = reinterpret_cast<float*>(smem_raw);  // [K][VS]
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const int col = 16 * c + lr;
+    const float m = __uint_as_float(colmax[col]) * S.dinv;
+#pragma unroll
+    for (int j = 0; j < RJ; ++j) {
+      int t2 = wave + WAVES * j;
+      if (t2 < Rt) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          int i = 16 * t2 + 4 * lk + q;
+          if (i < K) Y[(size_t)i * VS + col] = acc[j][c][q] * m;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int nval = min(NE, S.nt - e0);
+  for (int s = tid; s < nval * K; s += NT) {
+    int le = s / K, i = s % K;
+    yl[(size_t)(e0 + le) * K + i] = (i < Kl && i < 16 * Rt) ? Y[(size_t)i * VS + le] : 0.0f;
+  }
+}
+
+int gemm_h_ct(const DevSpace& S) { return S.Mp <= 64 ? 4 : 2; }
+
+size_t gemm_h_lds(const DevSpace& S) {
+  int NE = 16 * gemm_h_ct(S);
+  size_t vf = (size_t)3 * S.Mp * (NE + 1) * 4;
+  size_t vh = (size_t)2 * NE * S.R2 * 2;  // R2 = 3 Mh + 8
+  size_t y = (size_t)S.K * (NE + 1) * 4;
+  size_t m = vf > vh ? vf : vh;
+  return (m > y ? m : y) + (size_t)NE * 4;
+}
+
+bool mfma_h_supported(const DevSpace& S) {
+  const int NE = 16 * gemm_h_ct(S);
+  return S.Dh1hi != nullptr && S.xhhi != nullptr && S.K <= 16 * RMAX && S.Mp <= 128 && S.K32 <= 192 &&
+         3 * (S.Mh / 8) * NE <= GPT * NT && gemm_h_lds(S) <= 160 * 1024;
+}
+
+void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, float* yl, hipStream_t st) {
+  const int Kl = op.Kl > 0 ? op.Kl : S.K;
+  const bool timed = op.Kl == 0;
+  int th = timed ? timing_begin(TK_GATHER32, st) : -1;
+  hipLaunchKernelGGL(k_gather_tiles_h, dim3((unsigned)S.ntile), dim3(256), (size_t)S.K32 * TILE * 4, st, S, Kl, x,
+                     S.xhhi, S.xhlo, S.xhs, op.constrained);
+  timing_end(th, st);
+  const int CT = gemm_h_ct(S);
+  const int NE = 16 * CT;
+  int nb = (S.nt + NE - 1) / NE;
+  th = timed ? timing_begin(TK_ELEM32, st) : -1;
+  if (CT == 4)
+    hipLaunchKernelGGL(k_elem_gemm_h<4>, dim3(nb), dim3(NT), gemm_h_lds(S), st, S, op, S.xhhi, S.xhlo, S.xhs, yl);
+  else
+    hipLaunchKernelGGL(k_elem_gemm_h<2>, dim3(nb), dim3(NT), gemm_h_lds(S), st, S, op, S.xhhi, S.xhlo, S.xhs, yl);
+  timing_end(th, st);
+}
+
+}  // namespace tsem

@@ -234,6 +234,49 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   }
   upload(sp->Dp64, Dp, st);
   upload(sp->Dp32, std::vector<float>(Dp.begin(), Dp.end()), st);
+  // fp16-split factors: scaled by a power of two so that |D| < 1, each
+  // entry as hi + lo fp16; GEMM 1 layout [3][Mp][K32], GEMM 2 layout
+  // [K16][3 Mh] (transposed, components padded to Mh rows)
+  {
+    d.K32 = (K + 31) & ~31;
+    d.Mh = (Kd + 31) & ~31;
+    d.R2g = 3 * d.Mh;
+    d.R2 = 3 * d.Mh + 8;
+    const int K16 = (K + 15) & ~15;
+    double amax = 0;
+    for (double v : Dp) amax = std::max(amax, std::fabs(v));
+    int ex = 0;
+    std::frexp(amax > 0 ? amax : 1.0, &ex);
+    const double sD = std::ldexp(1.0, -ex);
+    d.dinv = (float)std::ldexp(1.0, ex);
+    auto split = [&](double v, uint16_t& hi, uint16_t& lo) {
+      float s = (float)(v * sD);
+      _Float16 h = (_Float16)s;
+      _Float16 l = (_Float16)(s - (float)h);
+      std::memcpy(&hi, &h, 2);
+      std::memcpy(&lo, &l, 2);
+    };
+    std::vector<uint16_t> h1((size_t)3 * d.Mp * d.K32, 0), l1(h1.size(), 0);
+    for (int a = 0; a < 3; ++a)
+      for (int m = 0; m < d.Mp; ++m)
+        for (int k = 0; k < K; ++k) {
+          size_t o = ((size_t)a * d.Mp + m) * d.K32 + k;
+          split(Dp[(size_t)(a * d.Mp + m) * d.Cp + k], h1[o], l1[o]);
+        }
+    std::vector<uint16_t> h2((size_t)K16 * d.R2g, 0), l2(h2.size(), 0);
+    for (int i = 0; i < K; ++i)
+      for (int a = 0; a < 3; ++a)
+        for (int m = 0; m < d.Mp; ++m) {
+          size_t o = (size_t)i * d.R2g + a * d.Mh + m;
+          split(Dp[(size_t)(a * d.Mp + m) * d.Cp + i], h2[o], l2[o]);
+        }
+    upload(sp->Dh1hi, h1, st); upload(sp->Dh1lo, l1, st);
+    upload(sp->Dh2hi, h2, st); upload(sp->Dh2lo, l2, st);
+    for (int t = 0; t < 16; ++t) {
+      d.kbeg1h[t] = d.kbeg1[t] & ~31;
+      d.kend2h[t] = d.kend2[t];
+    }
+  }
   upload(sp->dT32, std::vector<float>(dT.begin(), dT.end()), st);
   upload(sp->dN32, std::vector<float>(dN.begin(), dN.end()), st);
   upload(sp->mT32, std::vector<float>(mT.begin(), mT.end()), st);
@@ -270,6 +313,17 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   d.diag = sp->diag_d.as<double>(); d.invdiag = sp->invdiag.as<double>(); d.invdiag32 = sp->invdiag32.as<float>();
   d.Dp64 = sp->Dp64.as<double>(); d.Dp32 = sp->Dp32.as<float>();
   d.xt64 = sp->xt64.as<double>(); d.xt32 = sp->xt32.as<float>();
+  d.Dh1hi = sp->Dh1hi.as<uint16_t>(); d.Dh1lo = sp->Dh1lo.as<uint16_t>();
+  d.Dh2hi = sp->Dh2hi.as<uint16_t>(); d.Dh2lo = sp->Dh2lo.as<uint16_t>();
+  sp->xhhi.alloc((size_t)d.ntile * TILE * d.K32 * 2);
+  sp->xhlo.alloc((size_t)d.ntile * TILE * d.K32 * 2);
+  sp->xhs.alloc((size_t)d.ntile * TILE * 4);
+  d.xhhi = sp->xhhi.as<uint16_t>(); d.xhlo = sp->xhlo.as<uint16_t>(); d.xhs = sp->xhs.as<float>();
+  {
+    const int NE = 16 * (d.Mp <= 64 ? 4 : 2);
+    size_t vf = (size_t)3 * d.Mp * (NE + 1) * 4, vh = (size_t)2 * NE * d.R2 * 2, y = (size_t)K * (NE + 1) * 4;
+    d.hlds_v = std::max(std::max(vf, vh), y);
+  }
   // assembled diagonal on the device from the scatter map, host copy for
   // the boundary-condition setup
   sp->diag_d.alloc((size_t)S.ndof * 8);
@@ -749,7 +803,7 @@ int tsem_timing_read(double* out) {
 
 int tsem_set_apply_path(int path) {
   API_BEGIN
-  if (path < 0 || path > 1) throw std::invalid_argument("path must be 0 (auto) or 1 (scalar)");
+  if (path < 0 || path > 2) throw std::invalid_argument("path must be 0 (auto), 1 (scalar) or 2 (no fp16 split)");
   set_apply_path(path);
   API_END
 }

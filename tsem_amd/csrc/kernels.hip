@@ -334,7 +334,7 @@ void timing_end(int h, hipStream_t st) {
 
 static int apply_path_from_env() {
   const char* v = getenv("TSEM_APPLY_PATH");
-  return (v && v[0] == '1') ? 1 : 0;
+  return (v && (v[0] == '1' || v[0] == '2')) ? v[0] - '0' : 0;
 }
 static int g_apply_path = apply_path_from_env();
 void set_apply_path(int path) { g_apply_path = path; }
@@ -342,7 +342,9 @@ void set_apply_path(int path) { g_apply_path = path; }
 template <typename T>
 static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, double* partial, hipStream_t st) {
   const int kind = (sizeof(T) == 8) ? TK_GATHER64 : TK_GATHER32;
-  if (op.mass == 0.0 && g_apply_path == 0 && mfma_supported(S)) {
+  if (sizeof(T) == 4 && op.mass == 0.0 && g_apply_path == 0 && mfma_h_supported(S)) {
+    launch_apply_mfma_h(S, op, (const float*)x, (float*)yl, st);
+  } else if (op.mass == 0.0 && (g_apply_path == 0 || g_apply_path == 2) && mfma_supported(S)) {
     launch_apply_mfma<T>(S, op, x, yl, st);
   } else {
     constexpr int NE = 8;

@@ -76,6 +76,18 @@ struct DevSpace {
   // per 16-row tile: first column of GEMM 1, last row (exclusive) of GEMM 2
   int kbeg1[16] = {0};
   int kend2[16] = {0};
+  // fp16-split FP32 operator: factors scaled by 1/dinv and split hi/lo,
+  // Dh1[3][Mp][K32] (GEMM 1), Dh2[K16][3 Mh] (GEMM 2, D transposed, each
+  // component padded to Mh = roundup(Kd, 32) rows); element vectors
+  // xh[tile][K32 / 8][32][8] hi/lo with per-element scales xhs
+  int K32 = 0, Mh = 0, R2 = 0, R2g = 0;
+  size_t hlds_v = 0;  // LDS byte offset of the per-element maxima
+  float dinv = 1.0f;
+  int kbeg1h[16] = {0};
+  int kend2h[16] = {0};
+  const uint16_t *Dh1hi = nullptr, *Dh1lo = nullptr, *Dh2hi = nullptr, *Dh2lo = nullptr;
+  uint16_t *xhhi = nullptr, *xhlo = nullptr;
+  float* xhs = nullptr;
   // tiled scratch for gathered element vectors
   double* xt64 = nullptr;
   float* xt32 = nullptr;
@@ -101,6 +113,10 @@ void launch_apply(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, 
 template <typename T>
 void launch_apply_mfma(const DevSpace& S, const OpCoef& op, const T* x, T* yl, hipStream_t st);
 bool mfma_supported(const DevSpace& S);
+// fp16-split path for FP32 vectors (x: FP32 DOF vector, yl: element results)
+void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, float* yl, hipStream_t st);
+bool mfma_h_supported(const DevSpace& S);
+size_t gemm_h_lds(const DevSpace& S);
 // per-kernel HIP-event timing of full-order applies (kinds below); timing_read
 // fills {total ms, launches} per kind
 enum { TK_GATHER64 = 0, TK_ELEM64, TK_SCATTER64, TK_GATHER32, TK_ELEM32, TK_SCATTER32, TK_KINDS };
