@@ -162,6 +162,11 @@ int tsem_timing_read(double* out);
  * No counterpart. */
 int tsem_set_apply_path(int path);
 
+/* PCG iterations replayed as captured HIP graphs, one chunk of check_every
+ * iterations per graph (process wide; default on, TSEM_PCG_GRAPH=0 turns it
+ * off; kernel timing turns it off).  No counterpart. */
+int tsem_set_graphs(int enable);
+
 /* Assembled diagonal (device copy out, n_dof). */
 int tsem_diagonal(tsem_space* sp, double* diag_dev);
 

@@ -157,3 +157,29 @@ def test_dirichlet_reimposed_after_zero_elimination(ctx):
     xa, _ = a.pcg(ba, tol=1e-12)
     xb, _ = b.pcg(bb, tol=1e-12)
     assert np.abs(xa.cpu().numpy() - xb.cpu().numpy()).max() <= 1e-11 * np.abs(xa.cpu().numpy()).max()
+
+
+@pytest.mark.parametrize("precond", [1, 4])
+def test_graph_replays_match_eager(ctx, precond):
+    """PCG chunks replayed as captured graphs (many replays per solve, eager
+    copies and synchronizations between them, the cached graph reused by the
+    next solve) give the iterates of eager launches, bit for bit."""
+    from tsem_amd import lib
+    _say("graphs", precond)
+    prob = problems.Manufactured(1234)
+    xyz, tet = mesh.kuhn_cube(3, jitter=0.1, seed=13)
+    psp = api.Space(ctx, 5, xyz, tet, qorder=8)
+    b = api.poisson_setup(psp, prob)
+    res = {}
+    try:
+        for g in (0, 1, 1):
+            lib.check(lib.load().tsem_set_graphs(g))
+            for mixed in (False, True):
+                x, st = psp.pcg(b, tol=1e-11, precond=precond, mixed=mixed, check_every=7)
+                res.setdefault((g, mixed), []).append((x.cpu().numpy(), st["iterations"]))
+    finally:
+        lib.check(lib.load().tsem_set_graphs(1))
+    for mixed in (False, True):
+        x0, i0 = res[(0, mixed)][0]
+        for x1, i1 in res[(1, mixed)]:
+            assert i1 == i0 and np.array_equal(x1, x0)

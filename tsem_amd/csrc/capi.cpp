@@ -24,6 +24,12 @@ namespace tsem {
 thread_local std::string g_err;
 }
 
+// PCG iterations replayed as captured graphs (process wide; tsem_set_graphs)
+static bool g_pcg_graphs = [] {
+  const char* v = getenv("TSEM_PCG_GRAPH");
+  return !(v && v[0] == '0');
+}();
+
 const char* tsem_last_error(void) { return g_err.c_str(); }
 
 namespace {
@@ -801,6 +807,12 @@ int tsem_timing_read(double* out) {
   API_END
 }
 
+int tsem_set_graphs(int enable) {
+  API_BEGIN
+  g_pcg_graphs = enable != 0;
+  API_END
+}
+
 int tsem_set_apply_path(int path) {
   API_BEGIN
   if (path < 0 || path > 2) throw std::invalid_argument("path must be 0 (auto), 1 (scalar) or 2 (no fp16 split)");
@@ -872,14 +884,46 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
   int it = 0;
   static const bool trace = getenv("TSEM_PCG_TRACE") != nullptr;
   if (trace) fprintf(stderr, "[tsem pcg] start res %.3e bnorm %.3e rz %.3e\n", res, bnorm, hs.rz);
+  auto iteration = [&]() {
+    launch_apply<T>(d, op, p, q, yl, part, st);
+    launch_final_alpha(part, nb, sc, st);
+    update_and_precondition();
+    launch_p_update<T>(n, p, z, sc, st);
+  };
+  auto& G = (sizeof(T) == 8) ? sp->graph64 : sp->graph32;
   while (res >= tol && it <= max_iter) {
     int nrun = std::max(1, check_every);
-    for (int k = 0; k < nrun && it <= max_iter; ++k) {
-      launch_apply<T>(d, op, p, q, yl, part, st);
-      launch_final_alpha(part, nb, sc, st);
-      update_and_precondition();
-      launch_p_update<T>(n, p, z, sc, st);
-      ++it;
+    if (g_pcg_graphs && !timing_on() && it + nrun <= max_iter + 1) {
+      // the chunk of nrun iterations as one captured graph, replayed while
+      // the pointers and parameters it captured stay the same
+      std::vector<const void*> key = {x, r, z, p, q, yl, part, sc, sp->pmg.get(), (const void*)(intptr_t)precond,
+                                      (const void*)(intptr_t)smooth, (const void*)(intptr_t)nrun,
+                                      (const void*)(intptr_t)apply_path()};
+      if (!G.exec || G.key != key) {
+        G.reset();
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+          for (int k = 0; k < nrun; ++k) iteration();
+        } catch (...) {
+          (void)hipStreamEndCapture(st, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIPCHK(hipStreamEndCapture(st, &g));
+        hipError_t e = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) G.exec = nullptr;
+        HIPCHK(e);
+        G.key = key;
+      }
+      HIPCHK(hipGraphLaunch(G.exec, st));
+      it += nrun;
+    } else {
+      for (int k = 0; k < nrun && it <= max_iter; ++k) {
+        iteration();
+        ++it;
+      }
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(&hs, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st));

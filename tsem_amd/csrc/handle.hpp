@@ -84,6 +84,18 @@ struct tsem_space {
   DevBuf Dh1hi, Dh1lo, Dh2hi, Dh2lo, xhhi, xhlo, xhs;  // fp16-split FP32 operator
   // solver vectors
   DevBuf v64[6], v32[6];
+  // instantiated graph of one chunk of PCG iterations per precision, valid
+  // while every pointer and parameter it captured is unchanged (key)
+  struct PcgGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void*> key;
+    void reset() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      exec = nullptr;
+      key.clear();
+    }
+    ~PcgGraph() { reset(); }
+  } graph64, graph32;
   std::unique_ptr<Pmg> pmg;
   bool rhs_open = false;
   hipStream_t st() const {

@@ -338,6 +338,7 @@ static int apply_path_from_env() {
 }
 static int g_apply_path = apply_path_from_env();
 void set_apply_path(int path) { g_apply_path = path; }
+int apply_path() { return g_apply_path; }
 
 template <typename T>
 static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, double* partial, hipStream_t st) {
@@ -537,6 +538,21 @@ void launch_axpby(int n, T a, const T* x, T b, T* y, hipStream_t st) {
 }
 template void launch_axpby<double>(int, double, const double*, double, double*, hipStream_t);
 template void launch_axpby<float>(int, float, const float*, float, float*, hipStream_t);
+
+// zero fill as a kernel (not a memset node, so it can sit in captured graphs)
+template <typename T>
+__global__ void k_zero(int n, T* __restrict__ y) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i < n) y[i] = T(0);
+}
+template <typename T>
+void launch_zero(int n, T* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero<T>, dim3(grid1(n)), dim3(BLK), 0, st, n, y);
+}
+template void launch_zero<double>(int, double*, hipStream_t);
+template void launch_zero<float>(int, float*, hipStream_t);
+
+bool timing_on() { return g_kt.on; }
 
 template <typename T>
 __global__ void __launch_bounds__(BLK) k_dot(int n, const T* __restrict__ a, const T* __restrict__ b,

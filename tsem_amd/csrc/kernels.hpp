@@ -126,6 +126,7 @@ int timing_begin(int kind, hipStream_t st);  // -1 when timing is off
 void timing_end(int handle, hipStream_t st);
 // 0: auto (MFMA when supported), 1: force the scalar element kernel
 void set_apply_path(int path);
+int apply_path();
 template <typename T>
 void launch_scatter(const DevSpace& S, const T* yl, T* y, hipStream_t st);
 
@@ -145,6 +146,9 @@ void launch_scalar_beta(CgScalars* sc, hipStream_t st);
 
 template <typename T>
 void launch_axpby(int n, T a, const T* x, T b, T* y, hipStream_t st);
+template <typename T>
+void launch_zero(int n, T* y, hipStream_t st);  // y = 0 by a kernel (capturable)
+bool timing_on();
 template <typename T>
 void launch_dot(int n, const T* a, const T* b, double* partial, hipStream_t st);
 void launch_cast_f64_f32(int n, const double* x, double scale, float* y, hipStream_t st);

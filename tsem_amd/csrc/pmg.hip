@@ -164,7 +164,7 @@ void Pmg::smooth_level(int l, T* x, const T* b, int degree, bool zero_init, hipS
   double hi = 1.1 * lmax[l], lo = lmax[l] / 10.0;
   double theta = (hi + lo) / 2, delta = (hi - lo) / 2, sigma = theta / delta;
   double rho_old = 1.0 / sigma;
-  if (zero_init) HIPCHK(hipMemsetAsync(Ax, 0, (size_t)n * sizeof(T), st));
+  if (zero_init) launch_zero<T>(n, Ax, st);
   else apply_level<T>(l, x, Ax, st);
   // first step: d = Dinv r / theta
   hipLaunchKernelGGL(k_cheb<T>, dim3(nblk(n)), dim3(BLK), 0, st, n, b, (const T*)Ax, inv, T(0), T(1.0 / theta), dv, x);
@@ -186,7 +186,7 @@ void Pmg::vcycle(const T* r, T* z, int smooth, hipStream_t st) {
   X[0] = z;
   B[0] = const_cast<T*>(r);
   for (int l = 1; l < L; ++l) { X[l] = (T*)work[4 * l]; B[l] = (T*)work[4 * l + 1]; }
-  HIPCHK(hipMemsetAsync(z, 0, (size_t)n * sizeof(T), st));
+  launch_zero<T>(n, z, st);
   for (int l = 0; l + 1 < L; ++l) {
     smooth_level<T>(l, X[l], B[l], smooth, true, st);
     T* Ax = (T*)work[4 * l + 2];
@@ -195,7 +195,7 @@ void Pmg::vcycle(const T* r, T* z, int smooth, hipStream_t st) {
     launch_axpby<T>(n, T(1), B[l], T(-1), Ax, st);
     hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(n)), dim3(BLK), 0, st, n, (const uint8_t*)sumdof, order[l + 1],
                        (const T*)Ax, B[l + 1]);
-    HIPCHK(hipMemsetAsync(X[l + 1], 0, (size_t)n * sizeof(T), st));
+    launch_zero<T>(n, X[l + 1], st);
   }
   smooth_level<T>(L - 1, X[L - 1], B[L - 1], 2 * smooth + 2, true, st);
   for (int l = L - 2; l >= 0; --l) {

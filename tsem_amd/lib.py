@@ -69,6 +69,7 @@ SIGNATURES = {
     "tsem_scatter": [_vp, _vp, _vp],
     "tsem_diagonal": [_vp, _vp],
     "tsem_set_apply_path": [C.c_int],
+    "tsem_set_graphs": [C.c_int],
     "tsem_timing": [C.c_int],
     "tsem_timing_read": [_dp],
     "tsem_pcg": [_vp, _vp, _vp, C.POINTER(PcgOpts), C.POINTER(PcgStats)],
