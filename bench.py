@@ -235,13 +235,16 @@ def main():
     # per-kernel HIP-event timing of the mixed solves' full-order FP32 applies
     K, E, N = sp.K, sp.nt, sp.ndof
     Kd = args.p * (args.p + 1) * (args.p + 2) // 6
-    Kp = (K + 3) // 4 * 4
+    K32 = (K + 31) // 32 * 32
     nnz_sc = float(sp.nmap)
+    ct = 4 if (Kd + 15) // 16 * 16 <= 64 else 2
     algo = {
-        # k_gather_tiles<float>: x once, element descriptors (88 B) + geometry mask, tiled xt written
-        "k_gather_tiles<float>": ("hbm", 4.0 * N + 92.0 * E + 4.0 * E * Kp),
-        # k_elem_gemm<float, 4>: two GEMMs through the derivative factors, 12 Kd K flops per element
-        "k_elem_gemm<float, 4>": ("mfma", 12.0 * Kd * K * E),
+        # k_gather_tiles_h: x once, element descriptors (88 B) + geometry mask; the fp16 hi and lo halves
+        # of the scaled tile (2 x 2 B per padded coefficient) and one float scale per element written
+        "k_gather_tiles_h": ("hbm", 4.0 * N + 92.0 * E + 4.0 * E * K32 + 4.0 * E),
+        # k_elem_gemm_h<CT>: two GEMMs through the derivative factors, 12 Kd K FP32-equivalent flops per
+        # element (the fp16 MFMA work is three times that: hi*hi, hi*lo and lo*hi products)
+        "k_elem_gemm_h<%d>" % ct: ("mfma", 12.0 * Kd * K * E),
         # k_scatter<float>: element results, map codes + row pointers, y written, x and mask read (fused dot)
         "k_scatter<float>": ("hbm", 4.0 * E * K + 4.0 * nnz_sc + 4.0 * (N + 1) + 4.0 * N + 4.0 * N + 1.0 * N),
     }

@@ -135,6 +135,10 @@ int tsem_dirichlet(tsem_space* sp, const double* uv, const double* ue, const dou
 /* Zero Dirichlet elimination (operator only) for mark == flag:
  * impose_zero_boundary_condition(A, flag) (src/Discretize.cpp:1465-1502). */
 int tsem_zero_bc(tsem_space* sp, int flag);
+/* Zero the entries of a device vector on the DOFs of geometries with
+ * mark == flag: impose_zero_boundary_condition(Vector&, flag)
+ * (include/TetrahedralSEM.h:251, src/Discretize.cpp:1539-1551). */
+int tsem_zero_rows(tsem_space* sp, int flag, double* x_dev);
 
 /* Matrix-free operator, device: y = (stiff * S + mass * M) x with the
  * Dirichlet elimination when constrained != 0.  prec 64: double vectors,

@@ -120,6 +120,74 @@ def test_afepack_mesh_roundtrip(tmp_path):
         assert np.array_equal(a, b)
 
 
+_READER = r'''
+#include <cstdio>
+#include "tsem.hpp"
+int main(int, char** argv) {
+  tsem::RegularMesh m;
+  m.readMesh(argv[1]);
+  std::printf("%zu %zu %zu %zu\n", m.xyz.size() / 3, m.edge_vtx.size(), m.face_vtx.size(), m.tet.size() / 4);
+  for (double v : m.xyz) std::printf("%.17g ", v);
+  std::printf("\n");
+  for (int v : m.tet) std::printf("%d ", v);
+  std::printf("\n");
+  for (auto& e : m.edge_vtx) std::printf("%d %d ", e[0], e[1]);
+  std::printf("\n");
+  for (auto& f : m.face_vtx) std::printf("%d %d %d ", f[0], f[1], f[2]);
+  std::printf("\n");
+  for (auto* mk : {&m.mark_v, &m.mark_e, &m.mark_f}) {
+    for (int v : *mk) std::printf("%d ", v);
+    std::printf("\n");
+  }
+  return 0;
+}
+'''
+
+
+@pytest.mark.parametrize("name", ["cube_mixbc.mesh", "cube_dirichlet.mesh", "written"])
+def test_cxx_mesh_reader_matches_python(tmp_path, name):
+    """RegularMesh::readMesh of include/tsem.hpp against mesh.read_afepack_mesh
+    on the reference examples' cube meshes and on a written Kuhn mesh; the
+    file marks land on the same space geometries through both mappings."""
+    import subprocess
+    if name == "written":
+        xyz, tet = mesh.kuhn_cube(2, jitter=0.1, seed=4)
+        p = api.Space(None, 3, xyz, tet, qorder=5)
+        ev, fv, fe, tf = p.topology()
+        marks = mesh.geometry_marks(mesh.cube_plane_marks(xyz), ev, fv, 0b101111, 0b010000)
+        path = str(tmp_path / "k.mesh")
+        # reverse the file's edge and face order so file and space numbering differ
+        mesh.write_afepack_mesh(path, xyz, ev[::-1], fv[::-1], fe[::-1], tet, tf, (marks[0], marks[1][::-1],
+                                                                                      marks[2][::-1]))
+    else:
+        path = os.path.join(ROOT, "tests", "golden", name)
+    src = tmp_path / "reader.cpp"
+    src.write_text(_READER)
+    exe = str(tmp_path / "reader")
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                           "-I" + os.path.join(ROOT, "include"), str(src), "-o", exe])
+    out = subprocess.check_output([exe, path], text=True).split("\n")
+    nv, ne, nf, nt = (int(v) for v in out[0].split())
+    xyz, tet, marks, (ev, fv, _, _) = mesh.read_afepack_mesh(path)
+    assert (nv, ne, nf, nt) == (len(xyz), len(ev), len(fv), len(tet))
+    assert np.array_equal(np.array(out[1].split(), dtype=np.float64).reshape(-1, 3), xyz)
+    assert np.array_equal(np.array(out[2].split(), dtype=np.int64).reshape(-1, 4), tet)
+    assert np.array_equal(np.array(out[3].split(), dtype=np.int64).reshape(-1, 2), ev)
+    assert np.array_equal(np.array(out[4].split(), dtype=np.int64).reshape(-1, 3), fv)
+    for k in range(3):
+        assert np.array_equal(np.array(out[5 + k].split(), dtype=np.int64), marks[k])
+    # the mapping of file marks onto the space's own edges and faces
+    sp = api.Space(None, 3, xyz, tet, qorder=5)
+    mv, me, mf = sp.set_file_marks(marks, ev, fv)
+    sev, sfv, _, _ = sp.topology()
+    emap = {tuple(sorted(e)): m for e, m in zip(ev.tolist(), marks[1])}
+    fmap = {tuple(sorted(f)): m for f, m in zip(fv.tolist(), marks[2])}
+    assert all(me[i] == emap[tuple(sorted(e))] for i, e in enumerate(sev.tolist()))
+    assert all(mf[i] == fmap[tuple(sorted(f))] for i, f in enumerate(sfv.tolist()))
+    if name == "cube_mixbc.mesh":
+        assert sorted(np.bincount(mf).tolist()) == [2, 6, 10]
+
+
 def test_mesh_generators():
     xyz, tet = mesh.kuhn_cube(4, jitter=0.1, seed=2026)
     assert len(tet) == 6 * 64 and len(xyz) == 125

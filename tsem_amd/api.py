@@ -148,6 +148,19 @@ class Space:
         self.set_marks(mv, me, mf)
         return mv, me, mf
 
+    def set_file_marks(self, marks, edge_vtx, face_vtx):
+        """build_flag_bm from a mesh file's marks: per point, and per edge and
+        face of the file (edge_vtx / face_vtx point ids, read_afepack_mesh)
+        mapped onto this space's edges and faces by their vertex sets."""
+        mv, me_file, mf_file = marks
+        ev, fv, _, _ = self.topology()
+        emap = {tuple(sorted(int(v) for v in e)): int(m) for e, m in zip(edge_vtx, me_file)}
+        fmap = {tuple(sorted(int(v) for v in f)): int(m) for f, m in zip(face_vtx, mf_file)}
+        me = np.array([emap.get(tuple(sorted(int(v) for v in e)), 0) for e in ev], dtype=np.int32)
+        mf = np.array([fmap.get(tuple(sorted(int(v) for v in f)), 0) for f in fv], dtype=np.int32)
+        self.set_marks(np.asarray(mv, dtype=np.int32), me, mf)
+        return self.marks
+
     def bc_count(self, flag):
         n = np.zeros(3, np.int64)
         L.check(L.load().tsem_bc_count(self.h, flag, L.lptr(n)))
@@ -295,6 +308,12 @@ class Space:
 
     def zero_bc(self, flag):
         L.check(L.load().tsem_zero_bc(self.h, flag))
+
+    def zero_rows(self, flag, x):
+        """impose_zero_boundary_condition(Vector&, flag): x = 0 on the DOFs of
+        geometries with mark == flag (in place, device)."""
+        _ready()
+        L.check(L.load().tsem_zero_rows(self.h, flag, _ptr(x)))
 
     def apply(self, x, stiff=1.0, mass=0.0, constrained=False, out=None):
         import torch

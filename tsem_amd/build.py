@@ -3,7 +3,9 @@
     python -m tsem_amd.build          # or tsem_amd.build.build()
 
 Every source is compiled with hipcc for --offload-arch=gfx950; packed FP32
-instructions are disabled (-packed-fp32-ops) for all device code.
+instructions are disabled (-packed-fp32-ops) for all device code.  The C++
+example programs (examples/*.cpp, host code over include/tsem.hpp) are then
+linked against the library into examples/bin/.
 """
 import os
 import subprocess
@@ -15,6 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtsem_amd.so")
 SOURCES = ["refel.cpp", "space.cpp", "capi.cpp", "eigen.cpp", "quad_api.cpp", "kernels.hip", "apply_mfma.hip",
            "apply_mfma_h.hip", "pmg.hip", "block.hip", "quad.hip"]
+EXAMPLES = ["poisson_mixbc", "laplace_eig"]
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -40,9 +44,23 @@ def build(jobs=None, verbose=True):
     for o in objs:
         os.remove(o)
     os.rmdir(tmp)
+    build_examples()
     if verbose:
         print("built", LIB)
     return LIB
+
+
+def build_examples():
+    """Link the example programs against the in-tree library (rpath to it)."""
+    out = os.path.join(HERE, "examples", "bin")
+    os.makedirs(out, exist_ok=True)
+    for name in EXAMPLES:
+        exe = os.path.join(out, name)
+        subprocess.check_call(["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+                               "-I/opt/rocm/include", "-I" + INCLUDE, os.path.join(HERE, "examples", name + ".cpp"),
+                               "-o", exe + ".tmp", "-L" + HERE, "-ltsem_amd", "-L/opt/rocm/lib", "-lamdhip64",
+                               "-Wl,-rpath,$ORIGIN/../..:/opt/rocm/lib"])
+        os.replace(exe + ".tmp", exe)
 
 
 if __name__ == "__main__":

@@ -763,6 +763,26 @@ int tsem_zero_bc(tsem_space* sp, int flag) {
   API_END
 }
 
+int tsem_zero_rows(tsem_space* sp, int flag, double* x_dev) {
+  API_BEGIN
+  if (!sp || !x_dev) throw std::invalid_argument("null argument");
+  Space& S = sp->h;
+  if (S.mark[0].empty()) { g_err = "boundary marks not set"; return TSEM_ESTATE; }
+  std::vector<uint8_t> m(S.ndof, 0);
+  for (int dim = 0; dim <= 2; ++dim) {
+    int ng = dim == 0 ? S.topo.nv : dim == 1 ? S.topo.ne : S.topo.nf;
+    for (int g = 0; g < ng; ++g)
+      if (S.mark[dim][g] == flag)
+        for (int i = 0; i < S.ndg[dim]; ++i) m[S.geo_dof(dim, g) + i] = 1;
+  }
+  DevBuf md;
+  upload(md, m, sp->st());
+  launch_block_mask((int)S.ndof, 1, md.as<uint8_t>(), nullptr, x_dev, sp->st());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  API_END
+}
+
 int tsem_apply(tsem_space* sp, double stiff, double mass, int constrained, int prec, const void* x, void* y) {
   API_BEGIN
   if (!sp || !x || !y) throw std::invalid_argument("null argument");

@@ -64,6 +64,7 @@ SIGNATURES = {
     "tsem_neumann": [_vp, _dp, C.c_int, _vp],
     "tsem_dirichlet": [_vp, _dp, _dp, _dp, C.c_int, _vp, _dp],
     "tsem_zero_bc": [_vp, C.c_int],
+    "tsem_zero_rows": [_vp, C.c_int, _vp],
     "tsem_apply": [_vp, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _vp],
     "tsem_gather": [_vp, _vp, _vp],
     "tsem_scatter": [_vp, _vp, _vp],

@@ -143,7 +143,9 @@ def write_afepack_mesh(path, xyz, edge_vtx, face_vtx, face_edge, tet, tet_face, 
 
 
 def read_afepack_mesh(path):
-    """Parse an AFEPack ``.mesh`` file into (xyz, tet, marks, topology)."""
+    """Parse an AFEPack ``.mesh`` file into (xyz, tet, marks, topology):
+    marks per point, per file edge and per file face; topology = (edge_vtx,
+    face_vtx, face_edge, tet_face) in file order, vertices as point ids."""
     toks = open(path).read().split()
     pos = 0
 
@@ -174,5 +176,7 @@ def read_afepack_mesh(path):
     face_edge = np.array(geos[2][1], dtype=np.int64)
     tet = pt[np.array(geos[3][0], dtype=np.int64)].astype(np.int32)
     tet_face = np.array(geos[3][1], dtype=np.int64)
-    marks = (geos[0][2], geos[1][2], geos[2][2])
+    mark_pt = np.zeros(nv, dtype=np.int32)
+    mark_pt[pt] = geos[0][2]
+    marks = (mark_pt, geos[1][2], geos[2][2])
     return xyz, tet, marks, (edge_vtx, face_vtx, face_edge, tet_face)
