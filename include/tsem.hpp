@@ -16,7 +16,9 @@
 //
 // References: include/TetrahedralSEM.h:193-273, include/TSEMSolver.h:58-98.
 // Vectors live in device memory; host element access goes through
-// Vector::host() / Vector::assign() copies.  Errors throw std::runtime_error
+// Vector::host() / Vector::assign() copies.  Every Vector operation and every
+// library call is complete when it returns (the library runs on a
+// non-blocking stream of its context, so nothing else orders them).  Errors throw std::runtime_error
 // with tsem_last_error() (the reference prints and returns).
 #ifndef TSEM_AMD_HPP
 #define TSEM_AMD_HPP
@@ -125,7 +127,7 @@ class Vector {
   Vector& operator=(const Vector& o) {
     if (this != &o) {
       reinit(o.n_);
-      if (n_) hip_check(hipMemcpy(p_, o.p_, n_ * sizeof(T), hipMemcpyDeviceToDevice));
+      if (n_) copy_device(p_, o.p_, n_);
     }
     return *this;
   }
@@ -137,7 +139,16 @@ class Vector {
       n_ = n;
       if (n) hip_check(hipMalloc(&p_, n * sizeof(T)));
     }
-    if (n) hip_check(hipMemset(p_, 0, n * sizeof(T)));
+    if (n) {
+      hip_check(hipMemset(p_, 0, n * sizeof(T)));
+      hip_check(hipDeviceSynchronize());
+    }
+  }
+  // device-to-device copy, complete on return: the library works on its own
+  // non-blocking stream, which the null stream does not order against
+  static void copy_device(T* dst, const T* src, size_t n) {
+    hip_check(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice));
+    hip_check(hipDeviceSynchronize());
   }
   size_t size() const { return n_; }
   T* data() { return p_; }
@@ -401,16 +412,14 @@ class TSEMSolver {
     const int m = (int)x.size();
     const size_t n = (size_t)tsem.n_dof_total;
     Vector<T> X(n * m);
-    for (int j = 0; j < m; ++j)
-      hip_check(hipMemcpy(X.data() + j * n, x[j].data(), n * sizeof(T), hipMemcpyDeviceToDevice));
+    for (int j = 0; j < m; ++j) Vector<T>::copy_device(X.data() + j * n, x[j].data(), n);
     tsem_lobpcg_opts o{tol, 0, (int)max_step, 4, 2, A.stiff(), A.mass(), 0.0};
     tsem_lobpcg_stats s{};
     std::vector<double> lam(m);
     (void)B;
     int rc = tsem_lobpcg(tsem.space(), m, X.data(), lam.data(), &o, &s);
     if (rc != TSEM_ENOCONV) check(rc);
-    for (int j = 0; j < m; ++j)
-      hip_check(hipMemcpy(x[j].data(), X.data() + j * n, n * sizeof(T), hipMemcpyDeviceToDevice));
+    for (int j = 0; j < m; ++j) Vector<T>::copy_device(x[j].data(), X.data() + j * n, n);
     return lam;
   }
 

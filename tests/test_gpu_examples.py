@@ -93,7 +93,7 @@ def _python_poisson(ctx, path, M, tol=1e-14):
     return err, st["iterations"], n
 
 
-@pytest.mark.parametrize("M", [4, 6, 8])
+@pytest.mark.parametrize("M", [4, 6, 8, 10])
 def test_poisson_mixbc_example_matches_python(ctx, M):
     path = os.path.join(GOLD, "cube_mixbc.mesh")
     out = _run("poisson_mixbc", path, M)
@@ -104,12 +104,23 @@ def test_poisson_mixbc_example_matches_python(ctx, M):
     assert abs(err_c - err_p) <= 1e-12 + 1e-7 * err_p, (err_c, err_p)
 
 
-def test_poisson_mixbc_example_converges_spectrally():
+def test_poisson_mixbc_example_matches_oracle_errors():
+    """L2 errors at orders 4..10 against the oracle's on the same mesh and
+    data (tests/golden/mixbc_cube_l2.json, gen_mixbc_errors.py); the error
+    ratio per two orders grows (spectral convergence)."""
+    import json
+    gold = json.load(open(os.path.join(GOLD, "mixbc_cube_l2.json")))["errors"]
     path = os.path.join(GOLD, "cube_mixbc.mesh")
-    errs = [_field(_run("poisson_mixbc", path, M), "err_l2") for M in (4, 6, 8, 10)]
-    for a, b in zip(errs, errs[1:]):
-        assert b < a / 5, errs
-    assert errs[-1] < 1e-5, errs
+    errs = []
+    for M in (4, 6, 8, 10):
+        out = _run("poisson_mixbc", path, M)
+        err, n = _field(out, "err_l2"), int(_field(out, "n_dof"))
+        ref = gold[str(M)]
+        assert n == ref["n_dof"]
+        assert abs(err - ref["err_l2"]) <= 1e-12 + 1e-8 * ref["err_l2"], (M, err, ref)
+        errs.append(err)
+    ratios = [a / b for a, b in zip(errs, errs[1:])]
+    assert all(r > 3 for r in ratios) and ratios == sorted(ratios), (errs, ratios)
 
 
 def test_poisson_mixbc_example_on_written_mesh(ctx, tmp_path):

@@ -14,7 +14,9 @@ from tsem_amd import api, mesh, problems
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(2, 4, 0.1, 3), (3, 6, 0.15, 5), (2, 8, 0.1, 11), (2, 3, 0.1, 2)]
+# n^3 Kuhn cells, order M, jitter, seed; M = 10 and 12 (K = 286, 455) run the
+# LDS element kernel (the MFMA kernels take K <= 192)
+CASES = [(2, 4, 0.1, 3), (3, 6, 0.15, 5), (2, 8, 0.1, 11), (2, 3, 0.1, 2), (1, 10, 0.1, 4), (1, 12, 0.1, 9)]
 
 
 def _say(*a):

@@ -224,8 +224,10 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
       for (int m = 0; m < Kd; ++m)
         for (int i = 0; i < K; ++i)
           if (m >= lim(i)) Dp[(size_t)(a * d.Mp + m) * d.Cp + i] = 0.0;
+    // the tile tables serve the MFMA element kernels, which take K <= 192
+    // (M <= 8); higher orders run the LDS element kernel and leave them unused
     int Mt = (Kd + 15) / 16, Rt = (K + 15) / 16;
-    if (Mt > 16 || Rt > 16) throw std::invalid_argument("order too high for the tile tables");
+    if (Mt > 16 || Rt > 16) Mt = Rt = 0;
     for (int t = 0; t < Mt; ++t) {
       int kb = K;
       for (int i = 0; i < K; ++i)
