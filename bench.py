@@ -153,6 +153,10 @@ def main():
     ap.add_argument("--p", type=int, default=6)
     ap.add_argument("--qorder", type=int, default=30)
     ap.add_argument("--precond", type=int, default=1)
+    ap.add_argument("--mixed", type=int, default=1,
+                    help="mixed-precision form: 1 FP32 PCG with reliable updates, 2 restarted refinement")
+    ap.add_argument("--inner-tol", type=float, default=1e-3,
+                    help="mixed 1: residual reduction between reliable updates; 2: per inner solve")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-eigen", action="store_true", help="skip the LOBPCG section (16^3, p = 6, 32 pairs)")
@@ -185,8 +189,8 @@ def main():
         x = sp.zeros()
         torch.cuda.synchronize()
         t = time.perf_counter()
-        x, st = sp.pcg(b, x, tol=1e-10, criterion=1, precond=precond, mixed=mixed, inner_tol=1e-6,
-                       check_every=20, max_iter=200000)
+        x, st = sp.pcg(b, x, tol=1e-10, criterion=1, precond=precond, mixed=args.mixed if mixed else 0,
+                       inner_tol=args.inner_tol, check_every=20, max_iter=200000)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t
         if not st["converged"] or st["relres"] > 1e-10:
@@ -282,6 +286,8 @@ def main():
         "data": "synthetic (seeded jittered Kuhn mesh, manufactured solution)",
         "config": {"workload": "poisson_mixbc_kuhn%d_p%d" % (args.n, args.p), "tets": sp.nt, "dofs": sp.ndof,
                    "qorder": args.qorder, "precond": "jacobi" if args.precond == 1 else "pmg",
+                   "mixed_form": "reliable_updates" if args.mixed == 1 else "restarted_refinement",
+                   "inner_tol": args.inner_tol,
                    "jitter": 0.1, "mesh_seed": 2026, "problem_seed": 1234},
         "latency_ms": {"p50": float(np.percentile(times, 50)) * 1e3, "p99": float(np.percentile(times, 99)) * 1e3},
         "fp64_value": fp64_value,

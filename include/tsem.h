@@ -177,19 +177,26 @@ int tsem_diagonal(tsem_space* sp, double* diag_dev);
 typedef struct {
   double tol;          /* tolerance */
   int criterion;       /* 0: reference, ||r||_1 < tol (absolute); 1: ||b-Ax||_2/||b||_2 <= tol */
-  int max_iter;        /* iteration limit (outer for mixed) */
+  int max_iter;        /* iteration limit (mixed: limit on reliable updates / refinement steps) */
   int precond;         /* 1: Jacobi (reference default); 4: p-multigrid */
-  int mixed;           /* 0: FP64 PCG; 1: FP32 inner PCG inside FP64 iterative refinement */
-  double inner_tol;    /* mixed: relative reduction per inner solve (the last one stops
-                          once the outer tolerance is in reach, with a margin of 3) */
-  int inner_max;       /* mixed: inner iteration limit */
+  int mixed;           /* 0: FP64 PCG (criterion 0 or 1);
+                          1: FP32 PCG with reliable updates: the FP32 correction is added to
+                             the FP64 x and the FP64 true residual replaces the FP32 one each
+                             time the FP32 residual has fallen by inner_tol, search direction
+                             kept (criterion 1 only);
+                          2: FP64 iterative refinement around restarted FP32 PCG solves
+                             (criterion 1 only) */
+  double inner_tol;    /* mixed 1: residual reduction between reliable updates (clamped to
+                          [1e-6, 0.5]); mixed 2: relative reduction per inner solve (the last
+                          one stops once the outer tolerance is in reach, with a margin of 3) */
+  int inner_max;       /* mixed: FP32 iteration limit (mixed 1: in total; 2: per inner solve) */
   int smooth_steps;    /* p-MG smoother degree */
   int check_every;     /* host convergence check interval (iterations) */
 } tsem_pcg_opts;
 
 typedef struct {
   int iterations;      /* total (inner for mixed) PCG iterations */
-  int outer;           /* refinement steps (mixed) */
+  int outer;           /* reliable updates (mixed 1) or refinement steps (mixed 2) */
   double relres;       /* final true ||b-Ax||_2/||b||_2 (FP64) */
   double l1;           /* final ||r||_1 of the recursive residual */
   double seconds;      /* wall time of the solve (b and x resident) */

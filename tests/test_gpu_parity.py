@@ -112,20 +112,26 @@ def test_poisson_solution_matches_oracle(ctx, n, M, jit, seed):
     assert st2["converged"] and st2["l1"] < 1e-10
 
 
+@pytest.mark.parametrize("mode,inner_tol", [(1, 1e-2), (1, 1e-4), (2, 1e-4)])
 @pytest.mark.parametrize("precond", [1, 4])
-def test_mixed_precision_reaches_fp64_accuracy(ctx, precond):
-    import torch
-    _say("mixed", precond)
+def test_mixed_precision_reaches_fp64_accuracy(ctx, precond, mode, inner_tol):
+    """Both mixed forms reach the FP64 solve's accuracy; reliable updates keep
+    the FP32 recursion, so they need about the FP64 iteration count (with the
+    FP32 V-cycle only when the updates come at least every 10^-2 of residual
+    reduction: it drifts from conjugacy over longer stretches)."""
+    _say("mixed", precond, mode, inner_tol)
     prob = problems.Manufactured(1234)
     M = 5
     xyz, tet = mesh.kuhn_cube(3, jitter=0.1, seed=9)
     psp = api.Space(ctx, M, xyz, tet, qorder=M + 4)
     b = api.poisson_setup(psp, prob)
     x64, s64 = psp.pcg(b, tol=1e-12, criterion=1, precond=precond)
-    xm, sm = psp.pcg(b, tol=1e-12, criterion=1, precond=precond, mixed=True, inner_tol=1e-4)
-    assert s64["converged"] and sm["converged"] and sm["relres"] <= 1e-12
+    xm, sm = psp.pcg(b, tol=1e-12, criterion=1, precond=precond, mixed=mode, inner_tol=inner_tol, check_every=5)
+    assert s64["converged"] and sm["converged"] and sm["relres"] <= 1e-12, sm
     a, c = x64.cpu().numpy(), xm.cpu().numpy()
     assert np.linalg.norm(a - c) <= 1e-9 * np.linalg.norm(a)
+    if mode == 1 and (precond == 1 or inner_tol >= 1e-2):
+        assert sm["outer"] >= 1 and sm["iterations"] <= 1.15 * s64["iterations"] + 10, (sm, s64)
 
 
 def test_pmg_matches_jacobi_solution(ctx):

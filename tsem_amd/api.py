@@ -347,7 +347,7 @@ class Space:
             inner_max=100000, smooth_steps=2, check_every=1, raise_on_noconv=True):
         """solve_PCG on the eliminated operator; returns (x, stats dict)."""
         x = self.zeros() if x is None else x
-        o = L.PcgOpts(tol, criterion, max_iter, precond, 1 if mixed else 0, inner_tol, inner_max, smooth_steps,
+        o = L.PcgOpts(tol, criterion, max_iter, precond, int(mixed), inner_tol, inner_max, smooth_steps,
                       check_every)
         s = L.PcgStats()
         _ready()

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -859,9 +860,16 @@ namespace {
 // r = A x - b, y = P r, p = -y, alpha = r.y / p.Ap, x += alpha p,
 // r += alpha Ap, beta = r.y_new / r.y, p = beta p - y.
 // stop: criterion 0 -> |r|_1 < tol, 1 -> |r|_2 / bnorm < tol (recursive r).
+// Check hook: called with the scalars read back after every chunk of
+// iterations and a `refresh(restart)` that recomputes z = P r and r.z for the
+// current r (for a hook that replaced r) and, with restart, also resets the
+// search direction to p = -z; returns the residual measure tested against tol.
+using CheckHook = std::function<double(CgScalars&, const std::function<void(bool)>&)>;
+
 template <typename T>
 int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double bnorm, int max_iter, int precond,
-             int smooth, int check_every, int* iters, double* l1_out, double* l2_out) {
+             int smooth, int check_every, int* iters, double* l1_out, double* l2_out,
+             const CheckHook* hook = nullptr) {
   const DevSpace& d = sp->d;
   int n = d.ndof;
   hipStream_t st = sp->st();
@@ -950,7 +958,19 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(&hs, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    res = resid(hs);
+    if (hook) {
+      auto refresh = [&](bool restart) {
+        static const double zero = 0.0;
+        HIPCHK(hipMemcpyAsync(&sc->alpha, &zero, sizeof(double), hipMemcpyHostToDevice, st));
+        update_and_precondition();
+        if (restart) launch_axpby<T>(n, T(-1), z, T(0), p, st);
+        HIPCHK(hipMemcpyAsync(&hs, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+      };
+      res = (*hook)(hs, refresh);
+    } else {
+      res = resid(hs);
+    }
     if (trace)
       fprintf(stderr, "[tsem pcg%s] it %d res %.3e l1 %.3e l2 %.3e rz %.3e pq %.3e alpha %.3e beta %.3e\n",
               sizeof(T) == 8 ? "64" : "32", it, res, hs.l1, std::sqrt(hs.l2), hs.rz, hs.pq, hs.alpha, hs.beta);
@@ -988,6 +1008,7 @@ int tsem_pcg(tsem_space* sp, const double* b, double* x, const tsem_pcg_opts* o,
   API_BEGIN
   if (!sp || !b || !x || !o) throw std::invalid_argument("null argument");
   if (o->precond != 1 && o->precond != 4) throw std::invalid_argument("precond must be 1 (Jacobi) or 4 (p-MG)");
+  if (o->mixed < 0 || o->mixed > 2) throw std::invalid_argument("mixed must be 0, 1 or 2");
   if (!(o->tol > 0)) throw std::invalid_argument("tol must be positive");
   int n = sp->d.ndof;
   hipStream_t st = sp->st();
@@ -1010,8 +1031,73 @@ int tsem_pcg(tsem_space* sp, const double* b, double* x, const tsem_pcg_opts* o,
     s.outer = 1;
     s.l1 = l1;
     s.relres = true_relres(sp, b, x, r64, nullptr);
+  } else if (o->mixed == 1) {
+    // FP32 PCG with reliable updates: one FP32 Krylov recursion for the whole
+    // solve (vectors scaled by 1 / ||b||); whenever its residual has fallen by
+    // the factor inner_tol since the last update, and when it reports
+    // convergence, the FP32 correction is added to the FP64 x, the true
+    // residual b - A x is formed in FP64 and replaces the FP32 one.  The
+    // search direction is kept while the true residual is within a factor 2
+    // of the recursive one; a larger gap (the FP32 recursion has drifted)
+    // restarts it from p = -z, since the stale direction would then take a
+    // step scaled for the smaller residual
+    for (int k = 0; k < 4; ++k) sp->v32[k].alloc((size_t)n * 4);
+    sp->v32[4].alloc((size_t)n * 4);
+    sp->v32[5].alloc((size_t)n * 4);
+    float* rhs32 = sp->v32[4].as<float>();
+    float* d32 = sp->v32[5].as<float>();
+    float* r32 = sp->v32[0].as<float>();  // the residual pcg_core<float> iterates on
+    double rel = true_relres(sp, b, x, r64, &bnorm);
+    if (bnorm == 0) bnorm = 1;
+    const double delta = std::min(0.5, std::max(o->inner_tol, 1e-6));
+    int updates = 0, restarts = 0;
+    if (rel > o->tol) {
+      launch_cast_f64_f32(n, r64, 1.0 / bnorm, rhs32, st);
+      launch_zero<float>(n, d32, st);
+      double rmax = rel;
+      bool failed = false;
+      CheckHook hook = [&](CgScalars& hs, const std::function<void(bool)>& refresh) -> double {
+        const double rr = std::sqrt(hs.l2);  // recursive residual relative to ||b||
+        if (!std::isfinite(rr)) { failed = true; return rr; }
+        if (rr >= delta * rmax && rr >= o->tol) {
+          rmax = std::max(rmax, rr);
+          return rr;
+        }
+        launch_add_f32_f64(n, d32, bnorm, x, st);
+        launch_zero<float>(n, d32, st);
+        rel = true_relres(sp, b, x, r64, nullptr);
+        ++updates;
+        if (rel <= o->tol || !std::isfinite(rel) || updates > o->max_iter) {
+          failed = !(rel <= o->tol);
+          return rel <= o->tol ? rel : 0.0;  // 0 ends the loop
+        }
+        // the recursion's residual is A d - rhs = -(b - A x) / ||b||
+        launch_cast_f64_f32(n, r64, -1.0 / bnorm, r32, st);
+        const bool restart = rel > 2.0 * rr;
+        refresh(restart);
+        restarts += restart ? 1 : 0;
+        static const bool trace = getenv("TSEM_PCG_TRACE") != nullptr;
+        if (trace)
+          fprintf(stderr, "[tsem mixed] update %d: recursive %.3e true %.3e restarts %d\n", updates, rr, rel,
+                  restarts);
+        rmax = rel;
+        return rel;
+      };
+      int it = 0;
+      double l1 = 0, l2 = 0;
+      pcg_core<float>(sp, rhs32, d32, o->tol, 1, 1.0, o->inner_max, o->precond, o->smooth_steps, o->check_every,
+                      &it, &l1, &l2, &hook);
+      // an iteration limit leaves a correction behind
+      launch_add_f32_f64(n, d32, bnorm, x, st);
+      rel = true_relres(sp, b, x, r64, nullptr);
+      s.iterations = it;
+      if (failed || !(rel <= o->tol)) rc = TSEM_ENOCONV;
+    }
+    s.outer = updates;
+    s.relres = rel;
+    s.l1 = 0;
   } else {
-    // FP64 iterative refinement around FP32 inner PCG
+    // FP64 iterative refinement around restarted FP32 inner PCG
     for (int k = 0; k < 4; ++k) sp->v32[k].alloc((size_t)n * 4);
     sp->v32[4].alloc((size_t)n * 4);
     sp->v32[5].alloc((size_t)n * 4);
