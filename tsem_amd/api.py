@@ -343,9 +343,12 @@ class Space:
         L.check(L.load().tsem_diagonal(self.h, _ptr(d)))
         return d
 
-    def pcg(self, b, x=None, tol=1e-10, criterion=1, max_iter=100000, precond=1, mixed=False, inner_tol=1e-4,
+    def pcg(self, b, x=None, tol=1e-10, criterion=1, max_iter=100000, precond=1, mixed=False, inner_tol=1e-2,
             inner_max=100000, smooth_steps=2, check_every=1, raise_on_noconv=True):
-        """solve_PCG on the eliminated operator; returns (x, stats dict)."""
+        """solve_PCG on the eliminated operator; returns (x, stats dict).
+        mixed: 0 (False) FP64 PCG; 1 (True) FP32 PCG with reliable updates
+        every inner_tol of residual reduction; 2 FP64 refinement around
+        restarted FP32 solves to inner_tol each."""
         x = self.zeros() if x is None else x
         o = L.PcgOpts(tol, criterion, max_iter, precond, int(mixed), inner_tol, inner_max, smooth_steps,
                       check_every)

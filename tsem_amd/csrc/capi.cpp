@@ -6,6 +6,7 @@
 
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cmath>
@@ -1037,10 +1038,14 @@ int tsem_pcg(tsem_space* sp, const double* b, double* x, const tsem_pcg_opts* o,
     // the factor inner_tol since the last update, and when it reports
     // convergence, the FP32 correction is added to the FP64 x, the true
     // residual b - A x is formed in FP64 and replaces the FP32 one.  The
-    // search direction is kept while the true residual is within a factor 2
-    // of the recursive one; a larger gap (the FP32 recursion has drifted)
-    // restarts it from p = -z, since the stale direction would then take a
-    // step scaled for the smaller residual
+    // search direction is kept while the replaced residual differs from the
+    // recursive one by at most half its norm (measured, |r_rec - r_true|);
+    // a larger gap
+    // (the FP32 recursion has drifted) restarts it from p = -z, since the
+    // stale direction belongs to a different residual.  A breakdown of the
+    // FP32 recursion (non-finite residual) returns to the last FP64 state with
+    // a restarted direction and updates every 10^-2 of reduction (at most 3
+    // times)
     for (int k = 0; k < 4; ++k) sp->v32[k].alloc((size_t)n * 4);
     sp->v32[4].alloc((size_t)n * 4);
     sp->v32[5].alloc((size_t)n * 4);
@@ -1049,8 +1054,14 @@ int tsem_pcg(tsem_space* sp, const double* b, double* x, const tsem_pcg_opts* o,
     float* r32 = sp->v32[0].as<float>();  // the residual pcg_core<float> iterates on
     double rel = true_relres(sp, b, x, r64, &bnorm);
     if (bnorm == 0) bnorm = 1;
-    const double delta = std::min(0.5, std::max(o->inner_tol, 1e-6));
-    int updates = 0, restarts = 0;
+    double delta = std::min(0.5, std::max(o->inner_tol, 1e-6));
+    int updates = 0, restarts = 0, recoveries = 0;
+    float* p32 = sp->v32[2].as<float>();
+    float* q32 = sp->v32[3].as<float>();
+    CgScalars* scd = sp->scal.as<CgScalars>();
+    double* part = sp->partial.as<double>();
+    const int nbr = reduce_blocks(n);
+    static const bool trace = getenv("TSEM_PCG_TRACE") != nullptr;
     if (rel > o->tol) {
       launch_cast_f64_f32(n, r64, 1.0 / bnorm, rhs32, st);
       launch_zero<float>(n, d32, st);
@@ -1058,7 +1069,19 @@ int tsem_pcg(tsem_space* sp, const double* b, double* x, const tsem_pcg_opts* o,
       bool failed = false;
       CheckHook hook = [&](CgScalars& hs, const std::function<void(bool)>& refresh) -> double {
         const double rr = std::sqrt(hs.l2);  // recursive residual relative to ||b||
-        if (!std::isfinite(rr)) { failed = true; return rr; }
+        if (!std::isfinite(rr)) {
+          if (++recoveries > 3) { failed = true; return 0.0; }
+          launch_zero<float>(n, d32, st);
+          launch_zero<float>(n, p32, st);
+          launch_zero<float>(n, q32, st);
+          launch_cast_f64_f32(n, r64, -1.0 / bnorm, r32, st);
+          refresh(true);
+          ++restarts;
+          delta = std::max(delta, 1e-2);
+          rmax = rel;
+          if (trace) fprintf(stderr, "[tsem mixed] FP32 breakdown: back to true residual %.3e\n", rel);
+          return rel;
+        }
         if (rr >= delta * rmax && rr >= o->tol) {
           rmax = std::max(rmax, rr);
           return rr;
@@ -1071,15 +1094,21 @@ int tsem_pcg(tsem_space* sp, const double* b, double* x, const tsem_pcg_opts* o,
           failed = !(rel <= o->tol);
           return rel <= o->tol ? rel : 0.0;  // 0 ends the loop
         }
-        // the recursion's residual is A d - rhs = -(b - A x) / ||b||
+        // the recursion's residual is A d - rhs = -(b - A x) / ||b||; the gap
+        // |r_rec - r_true| before it is replaced
+        launch_gap(n, r32, r64, 1.0 / bnorm, part, st);
+        launch_reduce_final(part, nbr, 1, &scd->tmp[6], st);
+        double g2 = 0;
+        HIPCHK(hipMemcpyAsync(&g2, &scd->tmp[6], sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        const double gap = std::sqrt(g2);
         launch_cast_f64_f32(n, r64, -1.0 / bnorm, r32, st);
-        const bool restart = rel > 2.0 * rr;
+        const bool restart = !(gap <= 0.5 * rel);
         refresh(restart);
         restarts += restart ? 1 : 0;
-        static const bool trace = getenv("TSEM_PCG_TRACE") != nullptr;
         if (trace)
-          fprintf(stderr, "[tsem mixed] update %d: recursive %.3e true %.3e restarts %d\n", updates, rr, rel,
-                  restarts);
+          fprintf(stderr, "[tsem mixed] update %d: recursive %.3e true %.3e gap %.3e restarts %d\n", updates, rr,
+                  rel, gap, restarts);
         rmax = rel;
         return rel;
       };

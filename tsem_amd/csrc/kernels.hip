@@ -585,6 +585,23 @@ void launch_add_f32_f64(int n, const float* x, double s, double* y, hipStream_t 
   hipLaunchKernelGGL(k_addf, dim3(grid1(n)), dim3(BLK), 0, st, n, x, s, y);
 }
 
+// partial sums of |r32 + s r64|_2^2: how far an FP32 recursive residual has
+// drifted from the true one it is about to be replaced by
+__global__ void __launch_bounds__(BLK) k_gap(int n, const float* __restrict__ r32, const double* __restrict__ r64,
+                                             double s, double* __restrict__ partial) {
+  __shared__ double red[BLK];
+  double a = 0;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    double v = (double)r32[i] + s * r64[i];
+    a += v * v;
+  }
+  a = block_sum(a, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = a;
+}
+void launch_gap(int n, const float* r32, const double* r64, double s, double* partial, hipStream_t st) {
+  hipLaunchKernelGGL(k_gap, dim3(reduce_blocks(n)), dim3(BLK), 0, st, n, r32, r64, s, partial);
+}
+
 // r = b - Ax ; partial |r|_2^2 and |b|_2^2
 __global__ void __launch_bounds__(BLK) k_residual(int n, const double* __restrict__ b, const double* __restrict__ Ax,
                                                   double* __restrict__ r, double* __restrict__ partial) {

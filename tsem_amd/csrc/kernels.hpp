@@ -153,6 +153,8 @@ template <typename T>
 void launch_dot(int n, const T* a, const T* b, double* partial, hipStream_t st);
 void launch_cast_f64_f32(int n, const double* x, double scale, float* y, hipStream_t st);
 void launch_add_f32_f64(int n, const float* x, double scale, double* y, hipStream_t st);
+// partial[b] of |r32 + s r64|_2^2 over reduce_blocks(n) blocks
+void launch_gap(int n, const float* r32, const double* r64, double s, double* partial, hipStream_t st);
 void launch_residual(const DevSpace& S, int n, const double* b, const double* Ax, double* r, double* partial, hipStream_t st);
 
 // RHS: bl[e][i] = 6V/6 * sum_q w_q f[e][q] phi_q[i] for elements [e0, e1)
