@@ -249,8 +249,9 @@ def main():
         # k_elem_gemm_h<CT>: two GEMMs through the derivative factors, 12 Kd K FP32-equivalent flops per
         # element (the fp16 MFMA work is three times that: hi*hi, hi*lo and lo*hi products)
         "k_elem_gemm_h<%d>" % ct: ("mfma", 12.0 * Kd * K * E),
-        # k_scatter<float>: element results, map codes + row pointers, y written, x and mask read (fused dot)
-        "k_scatter<float>": ("hbm", 4.0 * E * K + 4.0 * nnz_sc + 4.0 * (N + 1) + 4.0 * N + 4.0 * N + 1.0 * N),
+        # k_scatter_chunks<float>: element results, map codes + row pointers, y written, x and mask read
+        # (fused dot)
+        "k_scatter_chunks<float>": ("hbm", 4.0 * E * K + 4.0 * nnz_sc + 4.0 * (N + 1) + 4.0 * N + 4.0 * N + 1.0 * N),
     }
     kernels = {}
     for i, name in enumerate(algo):

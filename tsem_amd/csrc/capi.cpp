@@ -294,6 +294,11 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   std::vector<int32_t> scp(S.sc_ptr.size());
   if (S.sc_ptr.back() >= (1LL << 31)) throw std::invalid_argument("scatter map too large");
   for (size_t i = 0; i < scp.size(); ++i) scp[i] = (int32_t)S.sc_ptr[i];
+  d.sc_chunk_max = 0;
+  for (int64_t g0 = 0; g0 < S.ndof; g0 += 256) {
+    int64_t g1 = std::min<int64_t>(S.ndof, g0 + 256);
+    d.sc_chunk_max = std::max(d.sc_chunk_max, (int)(S.sc_ptr[g1] - S.sc_ptr[g0]));
+  }
   upload(sp->sc_ptr, scp, st); upload(sp->sc_code, S.sc_code, st); upload(sp->sc_w, S.sc_w, st);
   upload(sp->sc_w32, std::vector<float>(S.sc_w.begin(), S.sc_w.end()), st);
   sp->cmask.assign(S.ndof, 0);

@@ -231,8 +231,6 @@ __device__ inline double block_sum(double v, double* red) {
   return r;
 }
 
-// one global DOF: fixed-order sum over the transposed gather map (4 loads in
-// flight; the additions stay in map order)
 // y[g] = the row's terms of the transposed gather map summed in map order
 // (or diag * x on eliminated rows).  Rows keep their natural order across
 // lanes, so neighbouring lanes often read the same element-result lines; each
@@ -285,6 +283,80 @@ __global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict
   if (!partial) return;
   double tot = block_sum(d, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// The same sums by chunks of SCH rows: a block first streams every term of
+// its chunk, one per lane (coalesced code loads, independent result loads,
+// the weight or sign applied exactly as above), into LDS, then each lane adds
+// its row's terms from LDS in map order.  Per-term products and the order of
+// the additions are those of k_scatter, so y is bit-identical; what changes
+// is that the loads no longer wait on each other row by row.  Chunks are
+// spread over XCDs like k_scatter's rows (XCD x owns a contiguous range).
+constexpr int SCH = BLK;
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __restrict__ yl, T* __restrict__ y,
+                                                        const T* __restrict__ xin, int constrained,
+                                                        double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  extern __shared__ unsigned char scat_raw[];
+  T* vals = reinterpret_cast<T*>(scat_raw);
+  __shared__ double red[BLK];
+  constexpr int U = 4;
+  const T* __restrict__ wt = (sizeof(T) == 8) ? (const T*)S.sc_w : (const T*)S.sc_w32;
+  double d = 0.0;
+  const int nch = (S.ndof + SCH - 1) / SCH;
+  const int nx = min((int)gridDim.x, NXCD);
+  const int xcd = blockIdx.x % NXCD, jb = blockIdx.x / NXCD;
+  const int nbx = gridDim.x / NXCD + (xcd < (int)(gridDim.x % NXCD) ? 1 : 0);
+  const int c0 = (int)((long long)xcd * nch / nx), c1 = (int)((long long)(xcd + 1) * nch / nx);
+  for (int c = c0 + jb; c < c1; c += nbx) {
+    const int g0 = c * SCH, g1 = min(S.ndof, g0 + SCH);
+    const int t0 = S.sc_ptr[g0], t1 = S.sc_ptr[g1];
+    for (int t = t0 + threadIdx.x; t < t1; t += U * BLK) {
+      int cc[U];
+      T v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) cc[u] = (t + u * BLK < t1) ? S.sc_code[t + u * BLK] : 0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = (t + u * BLK < t1) ? yl[cc[u] & 0x3fffffff] : T(0);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int tt = t + u * BLK;
+        if (tt < t1) {
+          T w = v[u];
+          if (cc[u] < 0) w = w * wt[tt];
+          else if (cc[u] & 0x40000000) w = -w;
+          vals[tt - t0] = w;
+        }
+      }
+    }
+    __syncthreads();
+    const int g = g0 + (int)threadIdx.x;
+    if (g < g1) {
+      const int a = S.sc_ptr[g] - t0, b = S.sc_ptr[g + 1] - t0;
+      T acc = T(0);
+      for (int t = a; t < b; ++t) acc += vals[t];
+      if (constrained && S.cmask[g]) acc = (T)S.diag[g] * xin[g];
+      y[g] = acc;
+      if (partial) d += (double)xin[g] * (double)acc;
+    }
+    __syncthreads();  // vals is refilled by the next chunk
+  }
+  if (!partial) return;
+  double tot = block_sum(d, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// the chunked scatter when a chunk's terms fit in LDS, else the row kernel
+template <typename T>
+static void scatter_dispatch(const DevSpace& S, const T* yl, T* y, const T* x, int constrained, double* partial,
+                             hipStream_t st) {
+  int ng = reduce_blocks(S.ndof);
+  size_t lds = (size_t)S.sc_chunk_max * sizeof(T);
+  if (S.sc_chunk_max > 0 && lds <= 64 * 1024)
+    hipLaunchKernelGGL(k_scatter_chunks<T>, dim3(ng), dim3(BLK), lds, st, S, yl, y, x, constrained, partial);
+  else
+    hipLaunchKernelGGL(k_scatter<T>, dim3(ng), dim3(BLK), 0, st, S, yl, y, x, constrained, partial);
 }
 
 // ---- optional per-kernel timing with HIP events on the launching stream ----
@@ -361,9 +433,8 @@ static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T*
     if (lds > 64 * 1024) throw std::invalid_argument("quadrature too fine for the potential operator (qorder <= 20)");
     hipLaunchKernelGGL(k_apply_potential<T>, dim3(S.nt), dim3(BLK), lds, st, S, op.pot, x, yl, op.constrained);
   }
-  int ng = reduce_blocks(S.ndof);
   int th = (op.Kl == 0) ? timing_begin(kind + 2, st) : -1;
-  hipLaunchKernelGGL(k_scatter<T>, dim3(ng), dim3(BLK), 0, st, S, yl, y, x, op.constrained, partial);
+  scatter_dispatch<T>(S, yl, y, x, op.constrained, partial, st);
   timing_end(th, st);
 }
 
@@ -380,8 +451,7 @@ void launch_apply<float>(const DevSpace& S, const OpCoef& op, const float* x, fl
 
 template <typename T>
 void launch_scatter(const DevSpace& S, const T* yl, T* y, hipStream_t st) {
-  int ng = reduce_blocks(S.ndof);
-  hipLaunchKernelGGL(k_scatter<T>, dim3(ng), dim3(BLK), 0, st, S, yl, y, (const T*)nullptr, 0, (double*)nullptr);
+  scatter_dispatch<T>(S, yl, y, (const T*)nullptr, 0, (double*)nullptr, st);
 }
 template void launch_scatter<double>(const DevSpace&, const double*, double*, hipStream_t);
 template void launch_scatter<float>(const DevSpace&, const float*, float*, hipStream_t);

@@ -32,6 +32,7 @@ inline int tile_count(int nt) { return 2 * ((nt + 63) / 64); }
 struct DevSpace {
   int nt = 0, K = 0, Kd = 0, M = 0, nface = 0;
   int ndof = 0;
+  int sc_chunk_max = 0;              // most scatter terms of any 256-row chunk
   const ElemDesc* desc = nullptr;
   const int32_t* slot = nullptr;     // per local mode: kind | geo << 2 | sub << 5
   const int32_t* ft_ptr = nullptr;   // face table CSR over (variant, face dof)
