@@ -95,6 +95,17 @@ __global__ void __launch_bounds__(256) k_gather_tiles(DevSpace S, int Kl, const 
   __shared__ int sm[TILE];
   const int tile = xcd_block(blockIdx.x, gridDim.x);
   const int e0 = tile * TILE;
+  // the face table rows of this space: w [R J], n [R], key [R J]
+  extern __shared__ unsigned char gft_raw[];
+  const int R = 6 * S.ndg2, J = S.ftc_J;
+  T* cw = reinterpret_cast<T*>(gft_raw);
+  int* cn = reinterpret_cast<int*>(cw + (size_t)R * J);
+  int* ck = cn + R;
+  for (int s = threadIdx.x; s < R * J; s += 256) {
+    cw[s] = (sizeof(T) == 8) ? (T)S.ftc_w[s] : (T)S.ftc_w32[s];
+    ck[s] = S.ftc_key[s];
+  }
+  for (int s = threadIdx.x; s < R; s += 256) cn[s] = S.ftc_n[s];
   {
     const int32_t* src = reinterpret_cast<const int32_t*>(S.desc + e0);
     int32_t* dst = reinterpret_cast<int32_t*>(sd);
@@ -125,13 +136,14 @@ __global__ void __launch_bounds__(256) k_gather_tiles(DevSpace S, int Kl, const 
         v = ((msk >> 14) & 1) ? T(0) : x[D.t + sub];
       } else {
         int fo = D.fo[geo];
-        int row = (fo & 7) * S.nface + sub;
+        int row = (fo & 7) * S.ndg2 + sub;
         const bool fmask = (msk >> (10 + geo)) & 1;
         T acc = T(0);
-        for (int t = S.ft_ptr[row]; t < S.ft_ptr[row + 1]; ++t) {
-          int key = S.ft_key[t];
+        const int nterm = cn[row];
+        for (int j = 0; j < nterm; ++j) {
+          int key = ck[row * J + j];
           int ie = key & 3, idx = key >> 2;
-          T w = (T)S.ft_w[t];
+          T w = cw[row * J + j];
           T xv;
           if (ie < 3) {
             int le2 = (fo >> (3 + 3 * ie)) & 7;
@@ -304,7 +316,8 @@ static void run(const DevSpace& S, const OpCoef& op, const T* Dp, T* xt, const T
   const bool timed = op.Kl == 0;
   const int kind = sizeof(T) == 8 ? TK_GATHER64 : TK_GATHER32;
   int th = timed ? timing_begin(kind, st) : -1;
-  hipLaunchKernelGGL(k_gather_tiles<T>, dim3((unsigned)S.ntile), dim3(256), 0, st, S, Kl, x, xt, op.constrained);
+  const size_t ftc = (size_t)6 * S.ndg2 * (S.ftc_J * (sizeof(T) + 4) + 4);
+  hipLaunchKernelGGL(k_gather_tiles<T>, dim3((unsigned)S.ntile), dim3(256), ftc, st, S, Kl, x, xt, op.constrained);
   timing_end(th, st);
   int nb = (S.nt + 16 * CT - 1) / (16 * CT);
   th = timed ? timing_begin(kind + 1, st) : -1;

@@ -59,8 +59,14 @@ __global__ void __launch_bounds__(256) k_gather_tiles_h(DevSpace S, int Kl, cons
   __shared__ ElemDesc sd[TILE];
   __shared__ int sm[TILE];
   __shared__ float smax[TILE];
-  extern __shared__ float vf[];  // [K32][TILE]
+  extern __shared__ float vf[];  // [K32][TILE], then the face table rows: w [R J], n [R], key [R J]
   const int tile = xcd_block(blockIdx.x, gridDim.x);
+  const int R = 6 * S.ndg2, J = S.ftc_J;
+  float* cw = vf + (size_t)S.K32 * TILE;
+  int* cn = reinterpret_cast<int*>(cw + (size_t)R * J);
+  int* ck = cn + R;
+  for (int s = threadIdx.x; s < R * J; s += 256) { cw[s] = S.ftc_w32[s]; ck[s] = S.ftc_key[s]; }
+  for (int s = threadIdx.x; s < R; s += 256) cn[s] = S.ftc_n[s];
   const int e0 = tile * TILE;
   const int nvalid = min(TILE, S.nt - e0);
   {
@@ -91,13 +97,14 @@ __global__ void __launch_bounds__(256) k_gather_tiles_h(DevSpace S, int Kl, cons
         v = ((msk >> 14) & 1) ? 0.0f : x[D.t + sub];
       } else {
         int fo = D.fo[geo];
-        int row = (fo & 7) * S.nface + sub;
+        int row = (fo & 7) * S.ndg2 + sub;
         const bool fmask = (msk >> (10 + geo)) & 1;
         float acc = 0.0f;
-        for (int t = S.ft_ptr[row]; t < S.ft_ptr[row + 1]; ++t) {
-          int key = S.ft_key[t];
+        const int nterm = cn[row];
+        for (int j = 0; j < nterm; ++j) {
+          int key = ck[row * J + j];
           int ie = key & 3, idx = key >> 2;
-          float w = (float)S.ft_w[t];
+          float w = cw[row * J + j];
           float xv;
           if (ie < 3) {
             int le2 = (fo >> (3 + 3 * ie)) & 7;
@@ -360,7 +367,8 @@ void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, fl
   const int Kl = op.Kl > 0 ? op.Kl : S.K;
   const bool timed = op.Kl == 0;
   int th = timed ? timing_begin(TK_GATHER32, st) : -1;
-  hipLaunchKernelGGL(k_gather_tiles_h, dim3((unsigned)S.ntile), dim3(256), (size_t)S.K32 * TILE * 4, st, S, Kl, x,
+  const size_t ftc = (size_t)6 * S.ndg2 * (S.ftc_J * 8 + 4);
+  hipLaunchKernelGGL(k_gather_tiles_h, dim3((unsigned)S.ntile), dim3(256), (size_t)S.K32 * TILE * 4 + ftc, st, S, Kl, x,
                      S.xhhi, S.xhlo, S.xhs, op.constrained);
   timing_end(th, st);
   const int CT = gemm_h_ct(S);

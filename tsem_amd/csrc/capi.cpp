@@ -191,6 +191,28 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
     fptr.push_back((int32_t)fkey.size());
   }
   upload(sp->ft_ptr, fptr, st); upload(sp->ft_key, fkey, st); upload(sp->ft_w, fw, st);
+  {
+    const int nd = S.ndg[2];
+    int J = 1;
+    for (int v = 0; v < 6; ++v)
+      for (int dd = 0; dd < nd; ++dd) J = std::max(J, (int)S.table.terms[(size_t)v * S.table.nface + dd].size());
+    std::vector<int32_t> cn((size_t)6 * nd, 0), ck((size_t)6 * nd * J, 0);
+    std::vector<double> cw((size_t)6 * nd * J, 0.0);
+    for (int v = 0; v < 6; ++v)
+      for (int dd = 0; dd < nd; ++dd) {
+        const auto& row = S.table.terms[(size_t)v * S.table.nface + dd];
+        const size_t r = (size_t)v * nd + dd;
+        cn[r] = (int32_t)row.size();
+        for (size_t j = 0; j < row.size(); ++j) {
+          ck[r * J + j] = row[j].ie | (row[j].idx << 2);
+          cw[r * J + j] = row[j].w;
+        }
+      }
+    upload(sp->ftc_n, cn, st); upload(sp->ftc_key, ck, st); upload(sp->ftc_w, cw, st);
+    upload(sp->ftc_w32, std::vector<float>(cw.begin(), cw.end()), st);
+    d.ndg2 = nd;
+    d.ftc_J = J;
+  }
   upload(sp->ghat, S.ghat, st);
   upload(sp->ghat32, std::vector<float>(S.ghat.begin(), S.ghat.end()), st);
   upload(sp->sixv, S.sixv, st);
@@ -318,6 +340,8 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   sp->partial.alloc((size_t)4 * (reduce_blocks((int)S.ndof) + 16) * sizeof(double));
   sp->scal.alloc(sizeof(CgScalars));
   d.desc = sp->desc.as<ElemDesc>(); d.slot = sp->slot.as<int32_t>();
+  d.ftc_n = sp->ftc_n.as<int32_t>(); d.ftc_key = sp->ftc_key.as<int32_t>();
+  d.ftc_w = sp->ftc_w.as<double>(); d.ftc_w32 = sp->ftc_w32.as<float>();
   d.ft_ptr = sp->ft_ptr.as<int32_t>(); d.ft_key = sp->ft_key.as<int32_t>(); d.ft_w = sp->ft_w.as<double>();
   d.ghat = sp->ghat.as<double>(); d.ghat32 = sp->ghat32.as<float>();
   d.sixv = sp->sixv.as<double>(); d.sixv32 = sp->sixv32.as<float>();

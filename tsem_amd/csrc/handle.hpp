@@ -76,6 +76,7 @@ struct tsem_space {
   std::vector<uint8_t> cmask;
   DevSpace d;
   // device storage
+  DevBuf ftc_n, ftc_key, ftc_w, ftc_w32;
   DevBuf desc, slot, ft_ptr, ft_key, ft_w, ghat, ghat32, sixv, sixv32, dT, dN, dT32, dN32, mT, mN, mT32, mN32;
   DevBuf sc_ptr, sc_code, sc_w, sc_w32, diag_d, invdiag, invdiag32, cmask_d, xe, pts3, wphi, Dp64, Dp32;
   DevBuf yl64, yl32, bl, partial, scal, xt64, xt32, gmask;

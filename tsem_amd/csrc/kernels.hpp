@@ -38,6 +38,14 @@ struct DevSpace {
   const int32_t* ft_ptr = nullptr;   // face table CSR over (variant, face dof)
   const int32_t* ft_key = nullptr;   // ie | idx << 2
   const double* ft_w = nullptr;
+  // the rows of the face table this space uses (variant v < 6, face mode
+  // d < ndg2), padded to ftc_J terms: row v ndg2 + d holds ftc_n terms
+  // ftc_key (ie | idx << 2) and ftc_w / ftc_w32; small enough for LDS
+  int ndg2 = 0, ftc_J = 0;
+  const int32_t* ftc_n = nullptr;
+  const int32_t* ftc_key = nullptr;
+  const double* ftc_w = nullptr;
+  const float* ftc_w32 = nullptr;
   const double* ghat = nullptr;      // nt x 6
   const float* ghat32 = nullptr;
   const double* sixv = nullptr;      // nt
