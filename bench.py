@@ -122,17 +122,20 @@ def serve_section(log, count=24, slots=8, seed=2026, load=0.8):
     reqs, stream = srv.run(serve.make_requests(count, seed=seed, rate=rate))
     srv.close()
     alone = serve.Server(slots=1)
-    iso = []
+    iso, iso_ph = [], []
     for r in serve.make_requests(count, seed=seed):
         alone.run([serve.Request(r.id, r.n, r.p, r.seed)])
         (done,), _ = alone.run([serve.Request(r.id, r.n, r.p, r.seed)])
         iso.append(done.done - done.start)
+        iso_ph.append(done.phases)
     alone.close()
     ratio = np.array([(r.done - r.arrival) / i for r, i in zip(reqs, iso)])
     for k in np.argsort(ratio)[::-1][:4]:
         r = reqs[k]
         log("serving worst: n %d p %d dofs %d wait %.3f s service %.3f s isolated %.3f s built %s worker %d"
             % (r.n, r.p, r.dofs, r.start - r.arrival, r.done - r.start, iso[k], r.built, r.worker))
+        log("  phases in the stream %s, alone %s" % ({a: round(v, 4) for a, v in (r.phases or {}).items()},
+                                                    {a: round(v, 4) for a, v in (iso_ph[k] or {}).items()}))
     out = dict(workload="served_poisson_n8-32_p3-8", requests=count, slots=slots,
                saturated={"dofs_per_s": sat["dofs_per_s"], "wall_s": sat["wall_s"],
                           "requests_per_s": count / sat["wall_s"]},

@@ -38,6 +38,23 @@ const char* tsem_last_error(void);
 /* Context: device + stream.  No counterpart (TSEMLib is host-only). */
 int tsem_ctx_create(int device, tsem_ctx** out);
 int tsem_ctx_destroy(tsem_ctx* ctx);
+/* Solve class of a context, for several contexts sharing one device: 0
+ * normal; 1 foreground; 2 background: a background tsem_pcg waits between
+ * its chunks of check_every iterations while a foreground tsem_pcg runs
+ * anywhere in the process, each time for at most as long as its previous
+ * chunk took (host-side cooperative scheduling; streams keep the default
+ * priority).  No counterpart (serving only). */
+int tsem_ctx_set_class(tsem_ctx* ctx, int cls);
+/* Explicit foreground span (any context): counts as foreground work until its
+ * end, e.g. around a whole request (setup and solve).  A background solve
+ * waits before each chunk while foreground work is active, for at most the
+ * duration of its previous chunk.  tsem_background_wait returns at once for
+ * a context that is not background, else when no foreground work is active
+ * or after max_seconds (<= 0: no limit): a wait point for host work of
+ * background requests. */
+int tsem_foreground_begin(tsem_ctx* ctx);
+int tsem_foreground_end(tsem_ctx* ctx);
+int tsem_background_wait(tsem_ctx* ctx, double max_seconds);
 
 /* Space of polynomial order M on a tetrahedral mesh; builds the index maps,
  * reference matrices, geometric factors and the assembled diagonal.

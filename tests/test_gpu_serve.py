@@ -44,3 +44,23 @@ def test_budget_serializes_and_evicts(ctx):
     assert all(spans[i + 1][0] >= spans[i][1] - 1e-6 for i in range(len(spans) - 1))
     assert s.in_use <= small
     s.close()
+
+
+def test_foreground_and_background_classes(ctx):
+    """With small topologies foreground and large ones background (large
+    solves pause between chunks while a small one runs), every request still
+    gets the bit-identical result it gets alone."""
+    _say("serve classes")
+    reqs = serve.make_requests(8, seed=5, n_range=(3, 6), p_range=(3, 6))
+    costs = sorted(serve.cost(r.n, r.p) for r in reqs)
+    split = costs[len(costs) // 2]
+    assert costs[0] <= split < costs[-1]
+    mixed = serve.Server(slots=4, small_slots=2, small_cost=split, keep_solutions=True)
+    got, summary = mixed.run(reqs)
+    mixed.close()
+    assert summary["max_relres"] <= 1e-10
+    alone = serve.Server(slots=1, keep_solutions=True, classes=False)
+    for r in serve.make_requests(8, seed=5, n_range=(3, 6), p_range=(3, 6)):
+        (one,), _ = alone.run([r])
+        assert np.array_equal(one.solution, got[r.id].solution), r
+    alone.close()

@@ -21,6 +21,23 @@ class Context:
         L.check(L.load().tsem_ctx_create(device, C.byref(h)))
         self.h = h
 
+    def set_class(self, cls):
+        """Solve class: 0 normal, 1 foreground, 2 background (a background
+        PCG pauses between chunks while a foreground PCG runs)."""
+        L.check(L.load().tsem_ctx_set_class(self.h, int(cls)))
+
+    def foreground_begin(self):
+        L.check(L.load().tsem_foreground_begin(self.h))
+
+    def foreground_end(self):
+        L.check(L.load().tsem_foreground_end(self.h))
+
+    def background_wait(self, max_seconds=0.0):
+        """Returns when no foreground span or solve is active, or after
+        max_seconds (<= 0: no limit); at once unless this context is
+        background.  The GIL is released while waiting."""
+        L.check(L.load().tsem_background_wait(self.h, float(max_seconds)))
+
     def close(self):
         if getattr(self, "h", None):
             try:
@@ -411,5 +428,7 @@ def poisson_setup(sp, problem, dirichlet_planes=0b000001, neumann_planes=0b11111
     sp.neumann(gq, 2, b)
     pv, pe, pf = sp.bc_points(1)
     sp.bval = sp.dirichlet(problem.u(pv), problem.u(pe), problem.u(pf), 1, b)
-    torch.cuda.synchronize()
+    # torch's own stream only: a device-wide synchronization would break a
+    # graph capture running in another thread
+    torch.cuda.current_stream().synchronize()
     return b

@@ -7,7 +7,9 @@
 #include <dlfcn.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cmath>
 #include <cstring>
@@ -33,6 +35,42 @@ static bool g_pcg_graphs = [] {
 }();
 
 const char* tsem_last_error(void) { return g_err.c_str(); }
+
+// Cooperative solve classes (process wide): while any foreground solve runs,
+// a background PCG solve waits between its chunks of iterations, each time
+// for at most as long as its previous chunk took: short foreground requests
+// get the device mostly alone, and a background solve keeps at least about
+// half of it.
+namespace {
+std::mutex g_fg_mu;
+std::condition_variable g_fg_cv;
+int g_fg_active = 0;
+
+struct ForegroundScope {
+  bool on;
+  explicit ForegroundScope(const tsem_ctx* c) : on(c && c->cls == 1) {
+    if (!on) return;
+    std::lock_guard<std::mutex> lk(g_fg_mu);
+    ++g_fg_active;
+  }
+  ~ForegroundScope() {
+    if (!on) return;
+    {
+      std::lock_guard<std::mutex> lk(g_fg_mu);
+      --g_fg_active;
+    }
+    g_fg_cv.notify_all();
+  }
+};
+
+// max_seconds <= 0: until no foreground work is active
+void background_wait(const tsem_ctx* c, double max_seconds) {
+  if (!c || c->cls != 2) return;
+  std::unique_lock<std::mutex> lk(g_fg_mu);
+  if (max_seconds <= 0) g_fg_cv.wait(lk, [] { return g_fg_active == 0; });
+  else g_fg_cv.wait_for(lk, std::chrono::duration<double>(max_seconds), [] { return g_fg_active == 0; });
+}
+}  // namespace
 
 namespace {
 // standard Jacobi P_n^{(a,b)}(x), a,b > -1
@@ -115,6 +153,41 @@ int tsem_ctx_create(int device, tsem_ctx** out) {
   c->device = device;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   *out = c;
+  API_END
+}
+
+int tsem_foreground_begin(tsem_ctx* ctx) {
+  API_BEGIN
+  if (!ctx) throw std::invalid_argument("null context");
+  std::lock_guard<std::mutex> lk(g_fg_mu);
+  ++g_fg_active;
+  API_END
+}
+
+int tsem_foreground_end(tsem_ctx* ctx) {
+  API_BEGIN
+  if (!ctx) throw std::invalid_argument("null context");
+  {
+    std::lock_guard<std::mutex> lk(g_fg_mu);
+    if (g_fg_active <= 0) throw std::logic_error("tsem_foreground_end without tsem_foreground_begin");
+    --g_fg_active;
+  }
+  g_fg_cv.notify_all();
+  API_END
+}
+
+int tsem_background_wait(tsem_ctx* ctx, double max_seconds) {
+  API_BEGIN
+  if (!ctx) throw std::invalid_argument("null context");
+  background_wait(ctx, max_seconds);
+  API_END
+}
+
+int tsem_ctx_set_class(tsem_ctx* ctx, int cls) {
+  API_BEGIN
+  if (!ctx) throw std::invalid_argument("null context");
+  if (cls < 0 || cls > 2) throw std::invalid_argument("class must be 0 (normal), 1 (foreground) or 2 (background)");
+  ctx->cls = cls;
   API_END
 }
 
@@ -951,32 +1024,47 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
     launch_p_update<T>(n, p, z, sc, st);
   };
   auto& G = (sizeof(T) == 8) ? sp->graph64 : sp->graph32;
+  double last_chunk = 0.0;  // host seconds of the previous chunk
   while (res >= tol && it <= max_iter) {
+    background_wait(sp->ctx, std::max(last_chunk, 1e-4));
+    const auto chunk_t0 = std::chrono::steady_clock::now();
     int nrun = std::max(1, check_every);
+    std::vector<const void*> key;
     if (g_pcg_graphs && !timing_on() && it + nrun <= max_iter + 1) {
       // the chunk of nrun iterations as one captured graph, replayed while
       // the pointers and parameters it captured stay the same
-      std::vector<const void*> key = {x, r, z, p, q, yl, part, sc, sp->pmg.get(), (const void*)(intptr_t)precond,
-                                      (const void*)(intptr_t)smooth, (const void*)(intptr_t)nrun,
-                                      (const void*)(intptr_t)apply_path()};
-      if (!G.exec || G.key != key) {
+      key = {x, r, z, p, q, yl, part, sc, sp->pmg.get(), (const void*)(intptr_t)precond,
+             (const void*)(intptr_t)smooth, (const void*)(intptr_t)nrun, (const void*)(intptr_t)apply_path()};
+      if ((!G.exec || G.key != key) && G.failed != key) {
+        // a capture can be invalidated from outside (another thread's
+        // device-wide synchronization): its launches never ran, so the chunk
+        // then runs eagerly and this key is not captured again
         G.reset();
         hipGraph_t g = nullptr;
-        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        try {
-          for (int k = 0; k < nrun; ++k) iteration();
-        } catch (...) {
-          (void)hipStreamEndCapture(st, &g);
+        bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+          try {
+            for (int k = 0; k < nrun; ++k) iteration();
+          } catch (...) {
+            ok = false;
+          }
+          if (hipStreamEndCapture(st, &g) != hipSuccess) ok = false;
+          if (ok && g && hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            G.exec = nullptr;
+            ok = false;
+          }
           if (g) (void)hipGraphDestroy(g);
-          throw;
         }
-        HIPCHK(hipStreamEndCapture(st, &g));
-        hipError_t e = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) G.exec = nullptr;
-        HIPCHK(e);
-        G.key = key;
+        if (ok && G.exec) {
+          G.key = key;
+        } else {
+          G.reset();
+          G.failed = key;
+          (void)hipGetLastError();
+        }
       }
+    }
+    if (!key.empty() && G.exec && G.key == key) {
       HIPCHK(hipGraphLaunch(G.exec, st));
       it += nrun;
     } else {
@@ -988,6 +1076,7 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(&hs, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    last_chunk = std::chrono::duration<double>(std::chrono::steady_clock::now() - chunk_t0).count();
     if (hook) {
       auto refresh = [&](bool restart) {
         static const double zero = 0.0;
@@ -1039,6 +1128,7 @@ int tsem_pcg(tsem_space* sp, const double* b, double* x, const tsem_pcg_opts* o,
   if (!sp || !b || !x || !o) throw std::invalid_argument("null argument");
   if (o->precond != 1 && o->precond != 4) throw std::invalid_argument("precond must be 1 (Jacobi) or 4 (p-MG)");
   if (o->mixed < 0 || o->mixed > 2) throw std::invalid_argument("mixed must be 0, 1 or 2");
+  ForegroundScope fg(sp->ctx);
   if (!(o->tol > 0)) throw std::invalid_argument("tol must be positive");
   int n = sp->d.ndof;
   hipStream_t st = sp->st();

@@ -65,6 +65,7 @@ struct tsem_ctx {
   hipStream_t stream = nullptr;
   int nspaces = 0;
   bool closing = false;
+  int cls = 0;  // solve class: 0 normal, 1 foreground, 2 background (tsem_ctx_set_class)
 };
 
 struct tsem_space {
@@ -90,6 +91,7 @@ struct tsem_space {
   struct PcgGraph {
     hipGraphExec_t exec = nullptr;
     std::vector<const void*> key;
+    std::vector<const void*> failed;  // key whose capture failed: run it eagerly
     void reset() {
       if (exec) (void)hipGraphExecDestroy(exec);
       exec = nullptr;

@@ -64,6 +64,10 @@ SIGNATURES = {
     "tsem_neumann": [_vp, _dp, C.c_int, _vp],
     "tsem_dirichlet": [_vp, _dp, _dp, _dp, C.c_int, _vp, _dp],
     "tsem_zero_bc": [_vp, C.c_int],
+    "tsem_ctx_set_class": [_vp, C.c_int],
+    "tsem_foreground_begin": [_vp],
+    "tsem_foreground_end": [_vp],
+    "tsem_background_wait": [_vp, C.c_double],
     "tsem_zero_rows": [_vp, C.c_int, _vp],
     "tsem_apply": [_vp, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _vp],
     "tsem_gather": [_vp, _vp, _vp],

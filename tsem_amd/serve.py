@@ -12,7 +12,9 @@ the vertex coordinates (tsem_space_set_geometry) and rebuilds its load
 vector and boundary data; idle cached spaces are dropped (least recently
 used first) when the budget needs their memory.  Workers are split into a
 small-request lane and a general lane, so small requests never queue behind
-large ones.
+large ones; and (classes=True) the contexts of small topologies are
+foreground, those of large ones background, so a large solve pauses between
+its chunks of iterations while a small one runs (tsem_ctx_set_class).
 
 Latency of a request = completion time - arrival time (queueing included);
 `Server.run` returns per-request records and p50/p99 summaries.
@@ -42,6 +44,7 @@ class Request:
     relres: float = None
     worker: int = -1
     built: bool = False           # the request's space had to be built
+    phases: dict = None           # seconds per phase: space, geometry, setup, solve
     solution: object = None
     error: str = None
 
@@ -85,7 +88,7 @@ def make_requests(count, seed=0, n_range=(8, 32), p_range=(3, 8), rate=None):
 
 class Server:
     def __init__(self, device=0, slots=8, small_slots=None, small_cost=None, hbm_budget=None, tol=1e-10,
-                 keep_solutions=False):
+                 keep_solutions=False, classes=True):
         """slots workers; the first small_slots of them (default half) take only
         requests whose cost is at most small_cost (default: the median over the
         request distribution), so small requests never queue behind large
@@ -102,6 +105,7 @@ class Server:
         self.budget = hbm_budget
         self.tol = tol
         self.keep = keep_solutions
+        self.classes = classes
         # cached spaces per (n, p), shared by all workers; each space has its
         # own context (stream) and serves one request at a time
         self.pool = {}                                # (n, p) -> [[space, ctx, busy, last use], ...]
@@ -167,6 +171,8 @@ class Server:
             if req is not None:
                 req.built = True
             ctx = api.Context(self.device)
+            if self.classes:
+                ctx.set_class(1 if cost(n, p) <= self.small_cost else 2)
             xyz, tet = mesh.kuhn_cube(n)
             sp = api.Space(ctx, p, xyz, tet, qorder=p + 3)
             with self.lock:
@@ -174,13 +180,32 @@ class Server:
         return ent[0]
 
     def solve_one(self, ent, req):
-        """Setup and solve of one request on the space of a pool entry."""
+        """Setup and solve of one request on the space of a pool entry.  A
+        foreground request holds a foreground span from its first device
+        work to its solution; a background one waits before each phase while
+        foreground work is active (up to 50 ms), and its solve pauses between
+        chunks for at most the previous chunk's duration."""
+        t0 = time.perf_counter()
         sp = self._space(ent, req.n, req.p, req)
-        xyz, _ = mesh.kuhn_cube(req.n, jitter=0.1, seed=req.seed)
-        sp.set_geometry(xyz)
-        b = api.poisson_setup(sp, problems.Manufactured(req.seed))
-        x, st = sp.pcg(b, tol=self.tol, criterion=1, precond=1, mixed=True, inner_tol=1e-2, check_every=10,
-                       max_iter=100000)
+        ctx = ent[1]
+        fg = self.classes and cost(req.n, req.p) <= self.small_cost
+        if fg:
+            ctx.foreground_begin()
+        try:
+            t1 = time.perf_counter()
+            ctx.background_wait(0.05)
+            xyz, _ = mesh.kuhn_cube(req.n, jitter=0.1, seed=req.seed)
+            sp.set_geometry(xyz)
+            t2 = time.perf_counter()
+            ctx.background_wait(0.05)
+            b = api.poisson_setup(sp, problems.Manufactured(req.seed))
+            t3 = time.perf_counter()
+            x, st = sp.pcg(b, tol=self.tol, criterion=1, precond=1, mixed=True, inner_tol=1e-2, check_every=10,
+                           max_iter=100000)
+        finally:
+            if fg:
+                ctx.foreground_end()
+        req.phases = dict(space=t1 - t0, geometry=t2 - t1, setup=t3 - t2, solve=time.perf_counter() - t3)
         req.dofs, req.iterations, req.relres = sp.ndof, st["iterations"], st["relres"]
         if self.keep:
             req.solution = x.cpu().numpy()
