@@ -108,19 +108,22 @@ def ks_section(ctx, log, n=24, p=5, steps=20, alpha=0.3):
     return out
 
 
-def serve_section(log, count=24, slots=8, seed=2026, load=0.8):
+def serve_section(log, count=24, slots=8, seed=2026, loads=(0.8, 0.5)):
     """Served Poisson stream: count requests (n in 8..32, p in 3..8, seeded
     jitter and problem), first all at once (saturated throughput, which also
-    fills the space pool), then with Poisson arrivals at `load` x that rate.
-    Each request's latency (arrival to completion) is set against its solve
-    alone on the device with its space already built (one worker, a warm-up
-    solve of the same request first)."""
+    fills the space pool), then with Poisson arrivals at each of `loads` x
+    that rate (the first is the headline).  Each request's latency (arrival to
+    completion) is set against its solve alone on the device with its space
+    already built (one worker, a warm-up solve of the same request first)."""
     from tsem_amd import serve
     srv = serve.Server(slots=slots)
     _, sat = srv.run(serve.make_requests(count, seed=seed))
-    rate = load * count / sat["wall_s"]
-    reqs, stream = srv.run(serve.make_requests(count, seed=seed, rate=rate))
+    runs = []
+    for load in loads:
+        rate = load * count / sat["wall_s"]
+        runs.append((load, rate) + srv.run(serve.make_requests(count, seed=seed, rate=rate)))
     srv.close()
+    _, rate, reqs, stream = runs[0]
     alone = serve.Server(slots=1)
     iso, iso_ph = [], []
     for r in serve.make_requests(count, seed=seed):
@@ -130,6 +133,12 @@ def serve_section(log, count=24, slots=8, seed=2026, load=0.8):
         iso_ph.append(done.phases)
     alone.close()
     ratio = np.array([(r.done - r.arrival) / i for r, i in zip(reqs, iso)])
+    by_load = {}
+    for load, rt, rq, stm in runs:
+        rr = np.array([(r.done - r.arrival) / i for r, i in zip(rq, iso)])
+        by_load[str(load)] = {"arrival_rate_per_s": rt, "dofs_per_s": stm["dofs_per_s"], "p50_ms": stm["p50_ms"],
+                              "p99_ms": stm["p99_ms"], "latency_over_isolated_p50": float(np.percentile(rr, 50)),
+                              "latency_over_isolated_p99": float(np.percentile(rr, 99))}
     for k in np.argsort(ratio)[::-1][:4]:
         r = reqs[k]
         log("serving worst: n %d p %d dofs %d wait %.3f s service %.3f s isolated %.3f s built %s worker %d"
@@ -142,7 +151,8 @@ def serve_section(log, count=24, slots=8, seed=2026, load=0.8):
                arrival_rate_per_s=rate, stream=stream,
                latency_over_isolated={"p50": float(np.percentile(ratio, 50)), "p99": float(np.percentile(ratio, 99)),
                                       "max": float(ratio.max())},
-               isolated_ms={"p50": float(np.percentile(iso, 50)) * 1e3, "p99": float(np.percentile(iso, 99)) * 1e3})
+               isolated_ms={"p50": float(np.percentile(iso, 50)) * 1e3, "p99": float(np.percentile(iso, 99)) * 1e3},
+               by_load=by_load)
     log("serving %s" % out)
     return out
 

@@ -88,6 +88,7 @@ class Space:
         """New vertex coordinates for the same connectivity (tsem_space_set_geometry);
         boundary eliminations and the potential are cleared."""
         self.xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self._bc_cache = {}
         _ready()
         L.check(L.load().tsem_space_set_geometry(self.h, L.dptr(self.xyz)))
 
@@ -109,12 +110,15 @@ class Space:
 
     # ---------------------------------------------------------------- host --
     def topology(self):
-        ev = np.zeros((self.ne, 2), np.int32)
-        fv = np.zeros((self.nf, 3), np.int32)
-        fe = np.zeros((self.nf, 3), np.int32)
-        tf = np.zeros((self.nt, 4), np.int32)
-        L.check(L.load().tsem_space_topology(self.h, L.iptr(ev), L.iptr(fv), L.iptr(fe), L.iptr(tf)))
-        return ev, fv, fe, tf
+        """(edge_vtx, face_vtx, face_edge, tet_face); fixed for the space, read once."""
+        if getattr(self, "_topo", None) is None:
+            ev = np.zeros((self.ne, 2), np.int32)
+            fv = np.zeros((self.nf, 3), np.int32)
+            fe = np.zeros((self.nf, 3), np.int32)
+            tf = np.zeros((self.nt, 4), np.int32)
+            L.check(L.load().tsem_space_topology(self.h, L.iptr(ev), L.iptr(fv), L.iptr(fe), L.iptr(tf)))
+            self._topo = (ev, fv, fe, tf)
+        return tuple(a.copy() for a in self._topo)
 
     def maps(self):
         off = np.zeros(self.nt * self.K + 1, np.int32)
@@ -154,7 +158,11 @@ class Space:
 
     def set_marks(self, mv, me, mf):
         """build_flag_bm(mesh): boundary mark per vertex, edge, face."""
-        self.marks = [np.ascontiguousarray(m, dtype=np.int32) for m in (mv, me, mf)]
+        new = [np.ascontiguousarray(m, dtype=np.int32) for m in (mv, me, mf)]
+        if self.marks is not None and all(np.array_equal(a, b) for a, b in zip(self.marks, new)):
+            return  # unchanged: the library state and the boundary point cache stay
+        self.marks = new
+        self._bc_cache = {}
         L.check(L.load().tsem_set_marks(self.h, *(L.iptr(m) for m in self.marks)))
 
     def set_cube_marks(self, dirichlet_planes=0b000001, neumann_planes=0b111110, lo=0.0, hi=1.0):
@@ -184,12 +192,18 @@ class Space:
         return [int(v) for v in n]
 
     def bc_points(self, flag):
-        """Sample points of boundary data on the geometries with mark flag."""
-        q = self.qorder
-        nv, ne, nf = self.bc_count(flag)
-        pv = np.zeros((nv, 3)); pe = np.zeros((ne, q, 3)); pf = np.zeros((nf, q * q, 3))
-        L.check(L.load().tsem_bc_points(self.h, flag, L.dptr(pv), L.dptr(pe), L.dptr(pf)))
-        return pv, pe, pf
+        """Sample points of boundary data on the geometries with mark flag
+        (kept until the geometry or the marks change)."""
+        cache = getattr(self, "_bc_cache", None)
+        if cache is None:
+            cache = self._bc_cache = {}
+        if flag not in cache:
+            q = self.qorder
+            nv, ne, nf = self.bc_count(flag)
+            pv = np.zeros((nv, 3)); pe = np.zeros((ne, q, 3)); pf = np.zeros((nf, q * q, 3))
+            L.check(L.load().tsem_bc_points(self.h, flag, L.dptr(pv), L.dptr(pe), L.dptr(pf)))
+            cache[flag] = (pv, pe, pf)
+        return tuple(a.copy() for a in cache[flag])
 
     def marked_faces(self, flag):
         return np.nonzero(self.marks[2] == flag)[0]
@@ -199,9 +213,10 @@ class Space:
         import torch
         return torch.zeros(self.ndof, dtype=dtype or torch.float64, device=self.device)
 
-    def rhs(self, f, chunk=None):
+    def rhs(self, f, chunk=None, pause=None):
         """calc_rhs: load vector of a source term f(points) evaluated on device
-        at the 3-D quadrature points, chunked over elements."""
+        at the 3-D quadrature points, chunked over elements (chunk elements
+        at a time; pause(), if given, runs between chunks)."""
         import torch
         lib = L.load()
         b = self.zeros()
@@ -209,6 +224,8 @@ class Space:
         if chunk is None:
             chunk = max(1, min(self.nt, int(2e8 // max(1, self.nq3 * 3))))
         for e0 in range(0, self.nt, chunk):
+            if pause is not None and e0 > 0:
+                pause()
             e1 = min(self.nt, e0 + chunk)
             pts = torch.empty((e1 - e0, self.nq3, 3), dtype=torch.float64, device=self.device)
             L.check(lib.tsem_qp_coords(self.h, e0, e1, _ptr(pts)))
@@ -413,21 +430,28 @@ class Space:
         return float(lam[0]), x, its.value
 
 
-def poisson_setup(sp, problem, dirichlet_planes=0b000001, neumann_planes=0b111110):
+def poisson_setup(sp, problem, dirichlet_planes=0b000001, neumann_planes=0b111110, chunk=None, pause=None):
     """Mixed-BC Poisson system on the unit cube, the sequence of
     examples/poisson_mixBC/poisson.cpp:74-89: load vector, Neumann load on
-    mark 2, Dirichlet elimination on mark 1.  Returns b (device)."""
+    mark 2, Dirichlet elimination on mark 1.  Returns b (device).  chunk and
+    pause go to Space.rhs."""
     import torch
     from .problems import cube_face_normal
     sp.set_cube_marks(dirichlet_planes, neumann_planes)
-    b = sp.rhs(lambda p: problem.f(p))
+    b = sp.rhs(lambda p: problem.f(p), chunk=chunk, pause=pause)
     ev, fv, _, _ = sp.topology()
     _, _, pf2 = sp.bc_points(2)
-    normals = cube_face_normal(sp.xyz[fv[sp.marked_faces(2)]])
-    gq = problem.dudn(pf2, [normals[:, None, d] for d in range(3)])
+
+    # boundary data evaluated on the device, handed to the library as host arrays
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(sp.device)
+
+    normals = dev(cube_face_normal(sp.xyz[fv[sp.marked_faces(2)]]))
+    gq = problem.dudn(dev(pf2), [normals[:, None, d] for d in range(3)]).cpu().numpy()
     sp.neumann(gq, 2, b)
     pv, pe, pf = sp.bc_points(1)
-    sp.bval = sp.dirichlet(problem.u(pv), problem.u(pe), problem.u(pf), 1, b)
+    uv, ue, uf = (problem.u(dev(q)).cpu().numpy() for q in (pv, pe, pf))
+    sp.bval = sp.dirichlet(uv, ue, uf, 1, b)
     # torch's own stream only: a device-wide synchronization would break a
     # graph capture running in another thread
     torch.cuda.current_stream().synchronize()

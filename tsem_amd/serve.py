@@ -198,7 +198,12 @@ class Server:
             sp.set_geometry(xyz)
             t2 = time.perf_counter()
             ctx.background_wait(0.05)
-            b = api.poisson_setup(sp, problems.Manufactured(req.seed))
+            # the load vector in pieces of about 2M points: torch work of all
+            # workers shares one stream, so no request queues much ahead of
+            # another; background requests pause between the pieces
+            chunk = max(1, int(2e6 // sp.nq3))
+            b = api.poisson_setup(sp, problems.Manufactured(req.seed), chunk=chunk,
+                                  pause=None if fg else (lambda: ctx.background_wait(0.01)))
             t3 = time.perf_counter()
             x, st = sp.pcg(b, tol=self.tol, criterion=1, precond=1, mixed=True, inner_tol=1e-2, check_every=10,
                            max_iter=100000)
