@@ -286,6 +286,12 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": kd["gbs"], "peak": 8000.0, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"] if roof["achieved"] else None
     roof["traffic"] = profile_traffic(dom, args.n, args.p)
+    # the whole mixed solve against HBM: algorithmic bytes of one FP32
+    # iteration (the three apply kernels, k_cg_update: x, r, p, q, 1/diag read
+    # and x, r, z written; k_p_update: p, z read, p written) x iterations / time
+    it_bytes = sum(v.get("algorithmic_bytes", 0.0) for v in kernels.values()) \
+        + (4.0 * E * K32 + 24.0 * E + 4.0 * E * K) + 32.0 * N + 12.0 * N
+    n_it = stats[-1]["iterations"]
     value = sp.ndof / mean
     fp64_value = max(v["value"] for v in fp64.values()) if fp64 else None
     out = {
@@ -314,6 +320,9 @@ def main():
         "fp64": fp64,
         "relres_mixed": stats[-1]["relres"],
         "roofline": roof,
+        "solve_hbm": {"bytes_per_fp32_iteration": it_bytes, "iterations": n_it,
+                      "achieved_GBps": it_bytes * n_it / mean / 1e9,
+                      "frac": it_bytes * n_it / mean / 8e12},
         "kernels": kernels,
         "cpu_baseline": None,
     }

@@ -163,6 +163,43 @@ class Vector {
     if (n_) hip_check(hipMemcpy(p_, h.data(), n_ * sizeof(T), hipMemcpyHostToDevice));
   }
 
+  // the deal.II Vector subset the reference's callers use
+  // (src/TSEMSolver.cpp:1848-1897, examples/*); host round trips, not for
+  // inner loops
+  T operator()(size_t i) const {
+    T v;
+    hip_check(hipMemcpy(&v, p_ + i, sizeof(T), hipMemcpyDeviceToHost));
+    return v;
+  }
+  void set(size_t i, T v) { hip_check(hipMemcpy(p_ + i, &v, sizeof(T), hipMemcpyHostToDevice)); }
+  T l1_norm() const {
+    T s = T(0);
+    for (T v : host()) s += std::fabs(v);
+    return s;
+  }
+  T l2_norm() const {
+    T s = T(0);
+    for (T v : host()) s += v * v;
+    return std::sqrt(s);
+  }
+  T operator*(const Vector& o) const {  // scalar product
+    std::vector<T> a = host(), b = o.host();
+    T s = T(0);
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+  }
+  Vector& operator*=(T a) {
+    std::vector<T> v = host();
+    for (T& x : v) x *= a;
+    assign(v);
+    return *this;
+  }
+  void add(T a, const Vector& o) {  // this += a o
+    std::vector<T> v = host(), w = o.host();
+    for (size_t i = 0; i < v.size(); ++i) v[i] += a * w[i];
+    assign(v);
+  }
+
  private:
   T* p_ = nullptr;
   size_t n_ = 0;
@@ -194,6 +231,15 @@ class SparseMatrix {
     return (size_t)info[0];
   }
   void copy_from(const SparseMatrix& o) { *this = o; }
+  size_t n() const { return m(); }
+  // diagonal entry of a stiffness operator (the eliminated form keeps the
+  // diagonal); the library holds the assembled stiffness diagonal only
+  T diag_element(size_t i) const {
+    if (mass_ != 0.0) throw std::invalid_argument("diag_element: stiffness operators only");
+    Vector<T> d(m());
+    check(tsem_diagonal(sp_, d.data()));
+    return T(stiff_) * d(i);
+  }
   tsem_space* space() const { return sp_; }
   double stiff() const { return stiff_; }
   double mass() const { return mass_; }

@@ -14,13 +14,6 @@
 
 static const double PI = 3.14159265358979323846;
 
-static double dot(const tsem::Vector<double>& a, const tsem::Vector<double>& b) {
-  std::vector<double> ha = a.host(), hb = b.host();
-  double s = 0;
-  for (size_t i = 0; i < ha.size(); ++i) s += ha[i] * hb[i];
-  return s;
-}
-
 int main(int argc, char* argv[]) {
   if (argc < 3) {
     std::printf("Usage: %s filename_mesh polynomial_order [m]\n", argv[0]);
@@ -58,9 +51,9 @@ int main(int argc, char* argv[]) {
       // Rayleigh quotient with the unconstrained operators (laplaceEig.cpp:89-95)
       tsem::Vector<double> tmpv(n_dof_total);
       tsem.stiff_matrix.vmult(tmpv, solution);
-      const double numerator = dot(tmpv, solution);
+      const double numerator = tmpv * solution;
       tsem.mass_matrix.vmult(tmpv, solution);
-      const double denominator = dot(tmpv, solution);
+      const double denominator = tmpv * solution;
       const double eig = numerator / denominator;
       std::printf("n_element = %d, M = %d, n_dof = %d, eig = %.16e, err_eig = %.6e\n", mesh.n_geometry(3), M,
                   n_dof_total, eig, std::fabs(eig - eig_ref));

@@ -12,6 +12,12 @@
 
 #include "tsem.hpp"
 
+// every member of the interface's class templates is compiled here
+template class tsem::Vector<double>;
+template class tsem::SparseMatrix<double>;
+template class tsem::TSEM<double>;
+template class tsem::TSEMSolver<double>;
+
 static const double PI = 3.14159265358979323846;
 
 // the exact solution and its data (poisson.cpp:25-27)
