@@ -348,14 +348,18 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
 
 int gemm_h_ct(const DevSpace& S) { return S.Mp <= 64 ? 4 : 2; }
 
-size_t gemm_h_lds(const DevSpace& S) {
+// LDS byte offset of the per-element maxima: after the largest of the FP32 V
+// tile, the fp16 V tile and the Y tile of one block
+size_t gemm_h_colmax_offset(const DevSpace& S) {
   int NE = 16 * gemm_h_ct(S);
   size_t vf = (size_t)3 * S.Mp * (NE + 1) * 4;
   size_t vh = (size_t)2 * NE * S.R2 * 2;  // R2 = 3 Mh + 8
   size_t y = (size_t)S.K * (NE + 1) * 4;
   size_t m = vf > vh ? vf : vh;
-  return (m > y ? m : y) + (size_t)NE * 4;
+  return m > y ? m : y;
 }
+
+size_t gemm_h_lds(const DevSpace& S) { return gemm_h_colmax_offset(S) + (size_t)16 * gemm_h_ct(S) * 4; }
 
 bool mfma_h_supported(const DevSpace& S) {
   const int NE = 16 * gemm_h_ct(S);

@@ -432,9 +432,7 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   sp->xhs.alloc((size_t)d.ntile * TILE * 4);
   d.xhhi = sp->xhhi.as<uint16_t>(); d.xhlo = sp->xhlo.as<uint16_t>(); d.xhs = sp->xhs.as<float>();
   {
-    const int NE = 16 * (d.Mp <= 64 ? 4 : 2);
-    size_t vf = (size_t)3 * d.Mp * (NE + 1) * 4, vh = (size_t)2 * NE * d.R2 * 2, y = (size_t)K * (NE + 1) * 4;
-    d.hlds_v = std::max(std::max(vf, vh), y);
+    d.hlds_v = gemm_h_colmax_offset(d);
   }
   // assembled diagonal on the device from the scatter map, host copy for
   // the boundary-condition setup

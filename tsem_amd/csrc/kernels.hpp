@@ -126,6 +126,7 @@ bool mfma_supported(const DevSpace& S);
 void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, float* yl, hipStream_t st);
 bool mfma_h_supported(const DevSpace& S);
 size_t gemm_h_lds(const DevSpace& S);
+size_t gemm_h_colmax_offset(const DevSpace& S);  // DevSpace::hlds_v
 // per-kernel HIP-event timing of full-order applies (kinds below); timing_read
 // fills {total ms, launches} per kind
 enum { TK_GATHER64 = 0, TK_ELEM64, TK_SCATTER64, TK_GATHER32, TK_ELEM32, TK_SCATTER32, TK_KINDS };
