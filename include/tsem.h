@@ -287,6 +287,12 @@ int tsem_integrate(tsem_space* sp, const double* f_dev, const double* g_dev, int
  * values at the points of those elements; the L2 error is the square root
  * of the sum over all chunks.  TSEM::calc_l2_error (src/CalcError.cpp:20-126). */
 int tsem_l2_error_chunk(tsem_space* sp, const double* x_dev, int e0, int e1, const double* u_dev, double* sum);
+/* Gradient error part of elements [e0, e1): *sum = sum_e V_e sum_q w_q
+ * |grad u_h - g|^2 with g the exact gradient at the points (device,
+ * (e1-e0) x n_qp3 x 3).  calc_l2_error_gradient(mesh, fem_space, sol, func)
+ * (include/TetrahedralSEM.h:276, src/CalcError.cpp:199-267); the error is
+ * the square root of the sum over all chunks. */
+int tsem_h1_error_chunk(tsem_space* sp, const double* x_dev, int e0, int e1, const double* g_dev, double* sum);
 
 /* LDA (unpolarized) at n points: Slater exchange + Perdew-Zunger 1981
  * correlation; vxc = d(rho eps_xc)/d rho, exc = rho eps_xc (device; either

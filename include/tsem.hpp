@@ -28,8 +28,10 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <fstream>
 #include <map>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -370,6 +372,86 @@ class TSEM {
     return std::sqrt(tot);
   }
 
+  // calc_l2_error_gradient(mesh, fem_space, x, func): func(p, g) writes the
+  // exact gradient at p into g[0..2]
+  double calc_l2_error_gradient(const RegularMesh&, const Vector<T>& x, void (*func)(double*, T*)) {
+    double tot = 0.0;
+    for_chunks([&](int e0, int e1, const std::vector<double>& pts, std::vector<double>& val) {
+      std::vector<double> g(3 * val.size());
+      for (size_t q = 0; q < val.size(); ++q) {
+        T gq[3];
+        func(const_cast<double*>(&pts[3 * q]), gq);
+        for (int d = 0; d < 3; ++d) g[3 * q + d] = (double)gq[d];
+      }
+      Vector<double> dg(g.size());
+      dg.assign(g);
+      double s = 0;
+      check(tsem_h1_error_chunk(sp_, x.data(), e0, e1, dg.data(), &s));
+      tot += s;
+    });
+    return std::sqrt(tot);
+  }
+
+  // write_coef(src, filename) / read_coef(dst, filename): 'M<TAB>n_dof', then
+  // one value per line (14 significant digits); read_coef takes, per
+  // geometry, the leading coefficients of a file of another order
+  // (src/Miscellaneous.cpp:34-125)
+  void write_coef(const std::vector<Vector<T>>& src, const std::string& filename) const {
+    std::ofstream out(filename);
+    if (!out) throw std::runtime_error("cannot write " + filename);
+    out << M_ << '\t' << n_dof_total << '\n';
+    char buf[64];
+    for (const auto& v : src)
+      for (T t : v.host()) {
+        std::snprintf(buf, sizeof buf, "%.14g\n", (double)t);
+        out << buf;
+      }
+  }
+  void write_coef(const Vector<T>& src, const std::string& filename) const {
+    write_coef(std::vector<Vector<T>>{src}, filename);
+  }
+  void read_coef(std::vector<Vector<T>>& dst, const std::string& filename) const {
+    std::ifstream in(filename);
+    if (!in) throw std::runtime_error("cannot open " + filename);
+    int m_in = 0;
+    long n_in = 0;
+    in >> m_in >> n_in;
+    auto binom = [](int n, int k) {
+      if (k < 0 || n < k) return 0L;
+      long r = 1;
+      for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+      return r;
+    };
+    int64_t info[12];
+    check(tsem_space_info(sp_, info));
+    const int ng[4] = {(int)info[2], (int)info[3], (int)info[4], (int)info[5]};
+    std::vector<int64_t> start((size_t)ng[0] + ng[1] + ng[2] + ng[3]);
+    check(tsem_space_dof_start(sp_, start.data()));
+    std::vector<int> dim(start.size());
+    for (size_t g = 0, o = 0; o < 4; ++o)
+      for (int i = 0; i < ng[o]; ++i) dim[g++] = (int)o;
+    std::vector<size_t> order(start.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return start[a] < start[b]; });
+    for (auto& v : dst) {
+      std::vector<T> src(n_in), out(n_dof_total, T(0));
+      for (auto& t : src) in >> t;
+      if (!in) throw std::runtime_error("truncated coefficient file " + filename);
+      long pos_in = 0;
+      for (size_t g : order) {
+        const long nd_in = binom(m_in - 1, dim[g]), keep = std::min(nd_in, binom(M_ - 1, dim[g]));
+        for (long i = 0; i < keep; ++i) out[start[g] + i] = src[pos_in + i];
+        pos_in += nd_in;
+      }
+      v.assign(out);
+    }
+  }
+  void read_coef(Vector<T>& dst, const std::string& filename) const {
+    std::vector<Vector<T>> one(1);
+    read_coef(one, filename);
+    dst = one[0];
+  }
+
   // calc_sum_dof: polynomial degree of every DOF
   void calc_sum_dof(std::vector<unsigned>& sum_dof) const {
     std::vector<int32_t> s(n_dof_total);
@@ -379,10 +461,11 @@ class TSEM {
 
  private:
   tsem_space* sp_ = nullptr;
-  int nt_ = 0, nq3_ = 0, qorder_ = 0;
+  int M_ = 0, nt_ = 0, nq3_ = 0, qorder_ = 0;
   std::vector<int32_t> face_vtx_, mark_f_;
 
   void init(int M, int qorder, RegularMesh& mesh, Context& ctx) {
+    M_ = M;
     qorder_ = qorder;
     check(tsem_space_create(ctx.get(), M, (int)(mesh.xyz.size() / 3), mesh.xyz.data(), (int)(mesh.tet.size() / 4),
                             mesh.tet.data(), nullptr, qorder, &sp_));

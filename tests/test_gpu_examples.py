@@ -51,6 +51,15 @@ def g(p):
     return -2 * PI * xp.cos(PI * p[..., 0]) * xp.sin(1.5 * PI * p[..., 1]) * xp.cos(0.25 * PI + 2 * PI * p[..., 2])
 
 
+def grad_u(p):
+    import torch
+    x, y, z = p[..., 0], p[..., 1], p[..., 2]
+    return torch.stack([-PI * torch.sin(PI * x) * torch.sin(1.5 * PI * y) * torch.sin(0.25 * PI + 2 * PI * z),
+                        1.5 * PI * torch.cos(PI * x) * torch.cos(1.5 * PI * y) * torch.sin(0.25 * PI + 2 * PI * z),
+                        2 * PI * torch.cos(PI * x) * torch.sin(1.5 * PI * y) * torch.cos(0.25 * PI + 2 * PI * z)],
+                       dim=-1)
+
+
 def _run(prog, *args):
     exe = os.path.join(BIN, prog)
     if not os.path.exists(exe):
@@ -88,20 +97,33 @@ def _python_poisson(ctx, path, M, tol=1e-14):
     sp.dirichlet(u(pv), u(pe), u(pf), 1, b)
     x, st = sp.pcg(b, tol=tol, criterion=0, precond=1, max_iter=100000, check_every=1)
     err = sp.l2_error(x, u)
+    _python_poisson.last = dict(err_l2g=sp.l2_error_gradient(x, grad_u), x=x.cpu().numpy())
     n = sp.ndof
     sp.close()
     return err, st["iterations"], n
 
 
 @pytest.mark.parametrize("M", [4, 6, 8, 10])
-def test_poisson_mixbc_example_matches_python(ctx, M):
+def test_poisson_mixbc_example_matches_python(ctx, M, tmp_path):
     path = os.path.join(GOLD, "cube_mixbc.mesh")
-    out = _run("poisson_mixbc", path, M)
+    coef = str(tmp_path / "sol.coef")
+    out = _run("poisson_mixbc", path, M, 1e-14, 1, coef)
     err_c, its_c, n_c = _field(out, "err_l2"), int(_field(out, "iterations")), int(_field(out, "n_dof"))
     err_p, its_p, n_p = _python_poisson(ctx, path, M)
     assert n_c == n_p
     assert abs(its_c - its_p) <= 2, (its_c, its_p)
     assert abs(err_c - err_p) <= 1e-12 + 1e-7 * err_p, (err_c, err_p)
+    # calc_l2_error_gradient through the C++ interface and through Python
+    g_c, g_p = _field(out, "err_l2g"), _python_poisson.last["err_l2g"]
+    assert abs(g_c - g_p) <= 1e-12 + 1e-7 * g_p, (g_c, g_p)
+    # write_coef: the C++ program's solution as text, read back here; the two
+    # solves (host- and device-evaluated load) agree to the solver accuracy
+    xyz, tet, _, _ = mesh.read_afepack_mesh(path)
+    sp = api.Space(None, M, xyz, tet, qorder=5)
+    xc = sp.read_coef(coef)
+    xp = _python_poisson.last["x"]
+    assert np.abs(xc - xp).max() <= 1e-8 * np.abs(xp).max()
+    sp.close()
 
 
 def test_poisson_mixbc_example_matches_oracle_errors():

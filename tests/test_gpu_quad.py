@@ -76,6 +76,29 @@ def test_l2_error_matches_oracle(pair):
     assert abs(got - ref) <= 1e-12 * ref
 
 
+def test_gradient_error_matches_oracle(pair):
+    """calc_l2_error_gradient against numpy: the oracle's gathered
+    coefficients and basis gradients at its points, mapped by J^-T of each
+    element (J columns P1-P0, P2-P0, P3-P0)."""
+    import torch
+    osp, psp = pair
+    prob = problems.Manufactured(77)
+    x = np.random.default_rng(8).standard_normal(psp.ndof) * 0.01
+    xl = osp.gather(x)
+    pts, w = osp.quad3()
+    _, dphi = osp.basis_at(pts)                      # (nq, K, 3)
+    qp = osp.qp_coords()
+    P = psp.xyz[psp.tet]                             # (nt, 4, 3)
+    J = np.stack([P[:, k + 1] - P[:, 0] for k in range(3)], axis=-1)   # J[e, d, k]
+    Ji = np.linalg.inv(J)                            # Ji[e, k, d]
+    gref = np.einsum("ei,qik->eqk", xl, dphi)
+    gh = np.einsum("eqk,ekd->eqd", gref, Ji)
+    ge = np.stack(prob.grad(qp), axis=-1)
+    ref = np.sqrt(np.sum(osp.volumes()[:, None] * w[None, :] * np.sum((gh - ge) ** 2, axis=-1)))
+    got = psp.l2_error_gradient(torch.from_numpy(x).cuda(), lambda p: torch.stack(prob.grad(p), dim=-1), chunk=5)
+    assert abs(got - ref) <= 1e-12 * ref, (got, ref)
+
+
 def test_moments_of_unit_cube(pair):
     import torch
     _, psp = pair

@@ -197,3 +197,35 @@ def test_mesh_generators():
     assert abs(det.sum() / 6 - 1.0) < 1e-12  # tets tile the cube
     g, t = mesh.graded_box(6)
     assert abs(g.min() + 10) < 1e-12 and abs(g.max() - 10) < 1e-12
+
+
+def test_coef_files_round_trip_and_change_order(tmp_path):
+    """write_coef/read_coef (src/Miscellaneous.cpp:34-125): the same order
+    round-trips to 14 digits; read into a higher order, the hierarchical
+    basis gives the same function (checked at points through the oracle);
+    read into a lower order, the leading modes are kept."""
+    xyz, tet = mesh.kuhn_cube(2, jitter=0.1, seed=5)
+    lo = api.Space(None, 4, xyz, tet, qorder=5)
+    hi = api.Space(None, 6, xyz, tet, qorder=5)
+    x = np.random.default_rng(2).standard_normal(lo.ndof)
+    path = str(tmp_path / "x.coef")
+    lo.write_coef(x, path)
+    head = open(path).readline().split()
+    assert head == ["4", str(lo.ndof)]
+    assert np.abs(lo.read_coef(path) - x).max() <= 1e-13 * np.abs(x).max()
+    xh = hi.read_coef(path)
+    o_lo, o_hi = O.OracleSpace(4, xyz, tet), O.OracleSpace(6, xyz, tet)
+    pts = np.random.default_rng(3).uniform(0, 0.25, size=(20, 3))
+    v_lo, _ = o_lo.basis_at(pts)
+    v_hi, _ = o_hi.basis_at(pts)
+    u_lo = o_lo.gather(x) @ v_lo.T
+    u_hi = o_hi.gather(xh) @ v_hi.T
+    assert np.abs(u_lo - u_hi).max() <= 1e-12 * np.abs(u_lo).max()
+    # back down: the order-6 vector read into order 4 gives x again
+    path2 = str(tmp_path / "xh.coef")
+    hi.write_coef(xh, path2)
+    assert np.abs(lo.read_coef(path2) - x).max() <= 1e-13 * np.abs(x).max()
+    # several vectors in one file
+    lo.write_coef([x, 2 * x], path)
+    a, b = lo.read_coef(path, count=2)
+    assert np.abs(b - 2 * a).max() <= 1e-12 * np.abs(b).max()

@@ -302,6 +302,59 @@ class Space:
             tot += s.value
         return math.sqrt(tot)
 
+    def l2_error_gradient(self, x, grad, chunk=None):
+        """calc_l2_error_gradient: ||grad u_h - grad u|| with grad a function
+        of the points (device) returning (..., 3)."""
+        import math
+        if chunk is None:
+            chunk = max(1, min(self.nt, int(2e8 // max(1, self.nq3 * 6))))
+        tot = 0.0
+        s = C.c_double()
+        for e0 in range(0, self.nt, chunk):
+            e1 = min(self.nt, e0 + chunk)
+            g = grad(self.qp_coords(e0, e1)).contiguous()
+            _ready()
+            L.check(L.load().tsem_h1_error_chunk(self.h, _ptr(x), e0, e1, _ptr(g), C.byref(s)))
+            tot += s.value
+        return math.sqrt(tot)
+
+    # ---- coefficient files (write_coef / read_coef, src/Miscellaneous.cpp:34-125)
+    def _geometry_layout(self):
+        """(first DOF, dimension) of every geometry, in DOF-layout order."""
+        start = self.dof_start()
+        dims = np.repeat(np.arange(4), [self.nv, self.ne, self.nf, self.nt])
+        order = np.argsort(start, kind="stable")
+        return start[order], dims[order]
+
+    def write_coef(self, x, path):
+        """Text file: 'M<TAB>n_dof', then one value per line (14 significant
+        digits); x (n_dof) or a list of such vectors, one after the other."""
+        vecs = x if isinstance(x, (list, tuple)) else [x]
+        with open(path, "w") as f:
+            f.write("%d\t%d\n" % (self.M, self.ndof))
+            for v in vecs:
+                v = v.cpu().numpy() if hasattr(v, "cpu") else np.asarray(v)
+                f.write("".join("%.14g\n" % t for t in v))
+
+    def read_coef(self, path, count=1):
+        """Vectors of this space from a coefficient file of order M' (<= or >=
+        M): per geometry the first C(min(M, M') - 1, dim) coefficients are
+        taken, the rest are zero (hierarchical basis).  Returns count host
+        arrays (one array when count == 1)."""
+        from math import comb
+        toks = open(path).read().split()
+        m_in, n_in = int(toks[0]), int(toks[1])
+        vals = np.array(toks[2:2 + count * n_in], dtype=np.float64).reshape(count, n_in)
+        start, dims = self._geometry_layout()
+        nd_in = np.array([comb(m_in - 1, d) for d in range(4)])
+        start_in = np.concatenate([[0], np.cumsum(nd_in[dims])[:-1]])
+        keep = np.minimum(nd_in[dims], np.array([comb(self.M - 1, d) for d in range(4)])[dims])
+        out = np.zeros((count, self.ndof))
+        for g in np.nonzero(keep)[0]:
+            k = keep[g]
+            out[:, start[g]:start[g] + k] = vals[:, start_in[g]:start_in[g] + k]
+        return out[0] if count == 1 else list(out)
+
     def lda_pz(self, rho):
         """LDA exchange (Slater) + Perdew-Zunger correlation: (v_xc, e_xc) at the points."""
         import torch

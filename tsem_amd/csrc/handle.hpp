@@ -82,7 +82,7 @@ struct tsem_space {
   DevBuf sc_ptr, sc_code, sc_w, sc_w32, diag_d, invdiag, invdiag32, cmask_d, xe, pts3, wphi, Dp64, Dp32;
   DevBuf yl64, yl32, bl, partial, scal, xt64, xt32, gmask;
   // quadrature-point work: basis at the points, weights, potential, scratch
-  DevBuf qphi, qphiT, qw, vpot, qxl, qpart, qout, grad2;
+  DevBuf qphi, qphiT, qdphi, qw, vpot, qxl, qpart, qout, grad2;
   DevBuf Dh1hi, Dh1lo, Dh2hi, Dh2lo, xhhi, xhlo, xhs;  // fp16-split FP32 operator
   // solver vectors
   DevBuf v64[6], v32[6];

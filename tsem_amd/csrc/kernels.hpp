@@ -75,6 +75,7 @@ struct DevSpace {
   const double* wq = nullptr;
   const double* phi = nullptr;
   const double* phiT = nullptr;
+  const double* dphi = nullptr;   // reference gradients at the points: [nq][K][3]
   const double* vpot = nullptr;
   const double* xe = nullptr;
   const double* pts = nullptr;    // per element: eliminated geometries (v0..3, e0..5, f0..3, interior)
@@ -200,6 +201,9 @@ void launch_integrate(const DevSpace& S, const double* f, const double* g, int n
                       hipStream_t st);
 // out = sum_{e in [e0, e1)} vol_e sum_q w_q (u_h(x_q) - u[(e - e0) nq + q])^2
 void launch_l2_chunk(const DevSpace& S, int e0, int e1, const double* xl, const double* u, double* partial,
+                     double* out, hipStream_t st);
+// gradient error over elements [e0, e1), g: exact gradient at the points ((e1-e0) x nq x 3)
+void launch_h1_chunk(const DevSpace& S, int e0, int e1, const double* xl, const double* g, double* partial,
                      double* out, hipStream_t st);
 // LDA, Slater exchange + Perdew-Zunger correlation (unpolarized): potential
 // v_xc and energy density e_xc = rho (eps_x + eps_c) per point

@@ -165,6 +165,69 @@ void launch_l2_chunk(const DevSpace& S, int e0, int e1, const double* xl, const 
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partial, nb, 1, out);
 }
 
+// sum_e V_e sum_q w_q |grad u_h - g|^2 over elements [e0, e1): grad u_h =
+// J^-T sum_i c_i grad_ref phi_i at the point, J = [P1-P0, P2-P0, P3-P0] the
+// element map of the quadrature coordinates (calc_l2_error_gradient,
+// src/CalcError.cpp:199-267)
+__global__ void __launch_bounds__(QB) k_h1_chunk(DevSpace S, int e0, int e1, const double* __restrict__ xl,
+                                                 const double* __restrict__ g, double* __restrict__ partial) {
+  extern __shared__ double sm[];
+  double* red = sm;        // QB
+  double* xs = sm + QB;    // K
+  const int nb = gridDim.x, nq = S.nq, K = S.K;
+  const int n = e1 - e0, per = (n + nb - 1) / nb;
+  const int ea = e0 + blockIdx.x * per, eb = min(e1, ea + per);
+  double acc = 0.0;
+  for (int e = ea; e < eb; ++e) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < K; i += QB) xs[i] = xl[(size_t)e * K + i];
+    __syncthreads();
+    const double* P = S.xe + (size_t)e * 12;
+    double J[3][3];  // J[d][k] = d x_d / d xref_k
+    for (int d = 0; d < 3; ++d)
+      for (int k = 0; k < 3; ++k) J[d][k] = P[3 * (k + 1) + d] - P[d];
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    double Ji[3][3];  // J^-1: Ji[k][d] = d xref_k / d x_d
+    Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+    Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+    Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    const double vol = S.sixv[e] / 6.0;
+    for (int q = threadIdx.x; q < nq; q += QB) {
+      double gr[3] = {0.0, 0.0, 0.0};
+      const double* dp = S.dphi + (size_t)q * K * 3;
+      for (int i = 0; i < K; ++i)
+        for (int k = 0; k < 3; ++k) gr[k] += dp[3 * i + k] * xs[i];
+      const double* ge = g + ((size_t)(e - e0) * nq + q) * 3;
+      double s = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const double gd = Ji[0][d] * gr[0] + Ji[1][d] * gr[1] + Ji[2][d] * gr[2] - ge[d];
+        s += gd * gd;
+      }
+      acc += vol * S.wq[q] * s;
+    }
+  }
+  __syncthreads();
+  double t = block_reduce(acc, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+void launch_h1_chunk(const DevSpace& S, int e0, int e1, const double* xl, const double* g, double* partial,
+                     double* out, hipStream_t st) {
+  int n = e1 - e0;
+  if (n <= 0) return;
+  int nb = n < IB ? n : IB;
+  hipLaunchKernelGGL(k_h1_chunk, dim3(nb), dim3(QB), (size_t)(QB + S.K) * 8, st, S, e0, e1, xl, g, partial);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partial, nb, 1, out);
+}
+
 // Perdew & Zunger, Phys. Rev. B 23, 5048 (1981), unpolarized fit to the
 // Ceperley-Alder correlation energy; Slater exchange eps_x = -(3/4)(3 rho/pi)^(1/3).
 __global__ void __launch_bounds__(QB) k_lda_pz(long n, const double* __restrict__ rho, double* __restrict__ vxc,

@@ -40,6 +40,23 @@ void ensure_quad(tsem_space* sp) {
   HIPCHK(hipStreamSynchronize(st));
 }
 
+// reference basis gradients at the 3-D points, built on first use
+void ensure_quad_grad(tsem_space* sp) {
+  ensure_quad(sp);
+  if (sp->qdphi.p) return;
+  const Space& S = sp->h;
+  const int K = S.K, nq = sp->nq3;
+  std::vector<double> dphi((size_t)nq * K * 3);
+  std::vector<long double> val(K), grad((size_t)K * 3);
+  for (int q = 0; q < nq; ++q) {
+    S.ref.eval(sp->p3[3 * q], sp->p3[3 * q + 1], sp->p3[3 * q + 2], val.data(), grad.data());
+    for (int i = 0; i < 3 * K; ++i) dphi[(size_t)q * K * 3 + i] = (double)grad[i];
+  }
+  upload(sp->qdphi, dphi, sp->st());
+  sp->d.dphi = sp->qdphi.as<double>();
+  HIPCHK(hipStreamSynchronize(sp->st()));
+}
+
 }  // namespace
 
 extern "C" {
@@ -99,6 +116,25 @@ int tsem_l2_error_chunk(tsem_space* sp, const double* x, int e0, int e1, const d
   double h = 0.0;
   if (e1 > e0) {
     launch_l2_chunk(sp->d, e0, e1, sp->qxl.as<double>(), u, sp->qpart.as<double>(), sp->qout.as<double>(), sp->st());
+    HIPCHK(hipMemcpyAsync(&h, sp->qout.p, 8, hipMemcpyDeviceToHost, sp->st()));
+  }
+  HIPCHK(hipStreamSynchronize(sp->st()));
+  *sum = h;
+  API_END
+}
+
+int tsem_h1_error_chunk(tsem_space* sp, const double* x, int e0, int e1, const double* g, double* sum) {
+  API_BEGIN
+  if (!sp || !x || !g || !sum) throw std::invalid_argument("null argument");
+  if (e0 < 0 || e1 > sp->d.nt || e0 > e1) throw std::invalid_argument("element range out of bounds");
+  ensure_quad_grad(sp);
+  sp->qxl.alloc((size_t)sp->d.nt * sp->d.K * 8);
+  sp->qpart.alloc((size_t)10 * integrate_blocks(sp->d) * 8);
+  sp->qout.alloc(10 * 8);
+  launch_gather(sp->d, x, sp->qxl.as<double>(), 0, sp->st());
+  double h = 0.0;
+  if (e1 > e0) {
+    launch_h1_chunk(sp->d, e0, e1, sp->qxl.as<double>(), g, sp->qpart.as<double>(), sp->qout.as<double>(), sp->st());
     HIPCHK(hipMemcpyAsync(&h, sp->qout.p, 8, hipMemcpyDeviceToHost, sp->st()));
   }
   HIPCHK(hipStreamSynchronize(sp->st()));

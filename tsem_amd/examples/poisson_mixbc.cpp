@@ -3,7 +3,7 @@
 // library: Dirichlet data u on mark 1, Neumann data g on mark 2, PCG with the
 // reference's absolute l1 stopping rule, L2 error against u.
 //
-//   poisson_mixbc mesh_file polynomial_order [tol] [precond]
+//   poisson_mixbc mesh_file polynomial_order [tol] [precond] [coef_file]
 //
 // precond: 1 Jacobi (the reference's default), 4 p-multigrid.
 #include <cmath>
@@ -26,10 +26,16 @@ double f(double* p) { return 7.25 * PI * PI * u(p); }
 double g(double* p) {
   return -2 * PI * std::cos(PI * p[0]) * std::sin(1.5 * PI * p[1]) * std::cos(0.25 * PI + 2 * PI * p[2]);
 }
+// grad u (poisson.cpp:28-33)
+void grad_u(double* p, double* dst) {
+  dst[0] = -PI * std::sin(PI * p[0]) * std::sin(1.5 * PI * p[1]) * std::sin(0.25 * PI + 2 * PI * p[2]);
+  dst[1] = 1.5 * PI * std::cos(PI * p[0]) * std::cos(1.5 * PI * p[1]) * std::sin(0.25 * PI + 2 * PI * p[2]);
+  dst[2] = 2 * PI * std::cos(PI * p[0]) * std::sin(1.5 * PI * p[1]) * std::cos(0.25 * PI + 2 * PI * p[2]);
+}
 
 int main(int argc, char* argv[]) {
   if (argc < 3) {
-    std::printf("Usage: %s filename_mesh polynomial_order [tol] [precond]\n", argv[0]);
+    std::printf("Usage: %s filename_mesh polynomial_order [tol] [precond] [coef_file]\n", argv[0]);
     return 1;
   }
   try {
@@ -56,8 +62,10 @@ int main(int argc, char* argv[]) {
     const int its = solver.solve_PCG(sp_matrix, solution, rhs, tol, 100000, precond, 1, 2);
 
     const double err_l2 = tsem.calc_l2_error(solution, &u);
-    std::printf("n_element = %d, M = %d, n_dof = %d, iterations = %d, err_l2 = %.16e\n", mesh.n_geometry(3), M,
-                n_dof_total, its, err_l2);
+    const double err_l2g = tsem.calc_l2_error_gradient(mesh, solution, &grad_u);
+    std::printf("n_element = %d, M = %d, n_dof = %d, iterations = %d, err_l2 = %.16e, err_l2g = %.16e\n",
+                mesh.n_geometry(3), M, n_dof_total, its, err_l2, err_l2g);
+    if (argc > 5) tsem.write_coef(solution, argv[5]);  // optional: the solution's coefficients
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 2;

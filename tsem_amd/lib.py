@@ -65,6 +65,7 @@ SIGNATURES = {
     "tsem_dirichlet": [_vp, _dp, _dp, _dp, C.c_int, _vp, _dp],
     "tsem_zero_bc": [_vp, C.c_int],
     "tsem_ctx_set_class": [_vp, C.c_int],
+    "tsem_h1_error_chunk": [_vp, _vp, C.c_int, C.c_int, _vp, C.POINTER(C.c_double)],
     "tsem_foreground_begin": [_vp],
     "tsem_foreground_end": [_vp],
     "tsem_background_wait": [_vp, C.c_double],
