@@ -255,6 +255,8 @@ def main():
     K32 = (K + 31) // 32 * 32
     nnz_sc = float(sp.nmap)
     ct = 4 if (Kd + 15) // 16 * 16 <= 64 else 2
+    slab = sp.slab_state()[0]
+    nch = (N + 255) // 256
     algo = {
         # k_gather_tiles_h: x once, element descriptors (88 B) + geometry mask; the fp16 hi and lo halves
         # of the scaled tile (2 x 2 B per padded coefficient) and one float scale per element written
@@ -262,10 +264,16 @@ def main():
         # k_elem_gemm_h<CT>: two GEMMs through the derivative factors, 12 Kd K FP32-equivalent flops per
         # element (the fp16 MFMA work is three times that: hi*hi, hi*lo and lo*hi products)
         "k_elem_gemm_h<%d>" % ct: ("mfma", 12.0 * Kd * K * E),
+    }
+    if slab:
+        # k_scatter_slab<float>: the entity slab (E K entries), per-DOF meta and per-chunk spans, y
+        # written, x and mask read (fused dot)
+        algo["k_scatter_slab<float>"] = ("hbm", 4.0 * E * K + 4.0 * N + 8.0 * nch + 4.0 * N + 4.0 * N + 1.0 * N)
+    else:
         # k_scatter_chunks<float>: element results, map codes + row pointers, y written, x and mask read
         # (fused dot)
-        "k_scatter_chunks<float>": ("hbm", 4.0 * E * K + 4.0 * nnz_sc + 4.0 * (N + 1) + 4.0 * N + 4.0 * N + 1.0 * N),
-    }
+        algo["k_scatter_chunks<float>"] = ("hbm", 4.0 * E * K + 4.0 * nnz_sc + 4.0 * (N + 1) + 4.0 * N + 4.0 * N
+                                           + 1.0 * N)
     kernels = {}
     for i, name in enumerate(algo):
         tot, cnt = kt[6 + 2 * i], kt[7 + 2 * i]
@@ -287,10 +295,12 @@ def main():
     roof["frac"] = roof["achieved"] / roof["peak"] if roof["achieved"] else None
     roof["traffic"] = profile_traffic(dom, args.n, args.p)
     # the whole mixed solve against HBM: algorithmic bytes of one FP32
-    # iteration (the three apply kernels, k_cg_update: x, r, p, q, 1/diag read
-    # and x, r, z written; k_p_update: p, z read, p written) x iterations / time
+    # iteration (the three apply kernels -- the element kernel reads the split
+    # tile and Ghat, writes E K results and with slabs reads 64 B of entity
+    # offsets per element --, k_cg_update: x, r, p, q, 1/diag read and x, r, z
+    # written; k_p_update: p, z read, p written) x iterations / time
     it_bytes = sum(v.get("algorithmic_bytes", 0.0) for v in kernels.values()) \
-        + (4.0 * E * K32 + 24.0 * E + 4.0 * E * K) + 32.0 * N + 12.0 * N
+        + (4.0 * E * K32 + 24.0 * E + 4.0 * E * K + (64.0 * E if slab else 0.0)) + 32.0 * N + 12.0 * N
     n_it = stats[-1]["iterations"]
     value = sp.ndof / mean
     fp64_value = max(v["value"] for v in fp64.values()) if fp64 else None
@@ -310,7 +320,7 @@ def main():
         "config": {"workload": "poisson_mixbc_kuhn%d_p%d" % (args.n, args.p), "tets": sp.nt, "dofs": sp.ndof,
                    "qorder": args.qorder, "precond": "jacobi" if args.precond == 1 else "pmg",
                    "mixed_form": "reliable_updates" if args.mixed == 1 else "restarted_refinement",
-                   "inner_tol": args.inner_tol,
+                   "inner_tol": args.inner_tol, "element_results": "entity_slab" if slab else "element_major",
                    "jitter": 0.1, "mesh_seed": 2026, "problem_seed": 1234},
         "latency_ms": {"p50": float(np.percentile(times, 50)) * 1e3, "p99": float(np.percentile(times, 99)) * 1e3},
         "fp64_value": fp64_value,

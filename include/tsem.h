@@ -100,6 +100,19 @@ int tsem_space_maps(const tsem_space* sp, int32_t* off, int32_t* ind, double* va
  * location_actualdof[location_geometry[.]] (src/BuildSpace.cpp:940-945). */
 int tsem_space_dof_start(const tsem_space* sp, int64_t* start);
 
+/* Entity-slab state: out[0] = 1 when the FP32 full-order operator of this
+ * space writes entity slabs (tables on the device, switch on, matrix-core
+ * element kernel with room for the tables), out[1] = slab entries (0: the
+ * mesh has no slab layout), out[2] = largest slab span of a 256-DOF chunk,
+ * out[3] = terms per transposed face row.  No counterpart. */
+int tsem_space_slab(const tsem_space* sp, int64_t* out);
+
+/* Host: y (n_dof) = the scatter of element-major results yl (n_elem x K)
+ * through the entity-slab tables of the space, with the device kernels'
+ * slab entry function (a check of the slab layout against tsem_space_maps).
+ * TSEM_ESTATE when the mesh has no slab layout.  No counterpart. */
+int tsem_space_slab_apply_host(const tsem_space* sp, const double* yl, double* y);
+
 /* Polynomial degree of each DOF: TSEM::calc_sum_dof (src/Miscellaneous.cpp:557). */
 int tsem_space_sum_dof(const tsem_space* sp, int32_t* sum_dof);
 
@@ -187,6 +200,15 @@ int tsem_set_apply_path(int path);
  * iterations per graph (process wide; default on, TSEM_PCG_GRAPH=0 turns it
  * off; kernel timing turns it off).  No counterpart. */
 int tsem_set_graphs(int enable);
+
+/* Element results written as entity slabs (process wide; default on,
+ * TSEM_SLAB=0 turns it off): the FP32 matrix-core element kernel stores each
+ * element's contribution to each of its global DOFs, already through the
+ * face re-expression, in per-geometry rows, and the scatter adds a DOF's
+ * rows in element order with no index codes or weights.  Used for the
+ * full-order operator without potential term; the sums run in another order
+ * than tsem_scatter's (same values to rounding).  No counterpart. */
+int tsem_set_slab(int enable);
 
 /* Assembled diagonal (device copy out, n_dof). */
 int tsem_diagonal(tsem_space* sp, double* diag_dev);

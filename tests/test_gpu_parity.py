@@ -80,6 +80,35 @@ def test_apply_matches_assembled(pair, coef, path):
         lib.check(lib.load().tsem_set_apply_path(0))
 
 
+def test_apply32_slab_matches_csr_scatter(pair):
+    """The FP32 operator with entity-slab element results (the default) and
+    with element-major results through the CSR scatter: same operator to
+    FP32 rounding (the sums run in another order), each deterministic."""
+    import torch
+    from tsem_amd import lib
+    osp, psp = pair
+    _say("slab", psp.M, psp.ndof)
+    L = lib.load()
+    active, total, _, jt = psp.slab_state()
+    # slab tables whenever the transposed face rows have at most 8 terms
+    # (orders up to 10); written by the matrix-core element kernel (K <= 192)
+    assert total == (psp.nt * psp.K if psp.M <= 10 else 0), (total, jt)
+    assert active == (psp.K <= 192)
+    x = torch.from_numpy(np.random.default_rng(5).standard_normal(psp.ndof)).float().cuda()
+    try:
+        out = {}
+        for slab in (1, 0):
+            lib.check(L.tsem_set_slab(slab))
+            a = psp.apply(x).double().cpu().numpy()
+            b = psp.apply(x).double().cpu().numpy()
+            assert np.array_equal(a, b)
+            out[slab] = a
+        scale = np.abs(out[0]).max()
+        assert np.abs(out[1] - out[0]).max() <= 2e-6 * scale
+    finally:
+        lib.check(L.tsem_set_slab(1))
+
+
 def test_apply_deterministic(pair):
     import torch
     osp, psp = pair

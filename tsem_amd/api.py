@@ -127,6 +127,21 @@ class Space:
         L.check(L.load().tsem_space_maps(self.h, L.iptr(off), L.iptr(ind), L.dptr(val)))
         return off, ind, val
 
+    def slab_state(self):
+        """(active, slab entries, largest chunk span, face row terms): active
+        when the FP32 operator writes entity slabs (tsem_space_slab)."""
+        out = np.zeros(4, np.int64)
+        L.check(L.load().tsem_space_slab(self.h, L.lptr(out)))
+        return bool(out[0]), int(out[1]), int(out[2]), int(out[3])
+
+    def slab_apply_host(self, yl):
+        """Scatter of element-major results yl (nt x K) through the entity-slab
+        tables, on the host (a check of the layout the device kernels use)."""
+        yl = np.ascontiguousarray(yl, dtype=np.float64).reshape(self.nt, self.K)
+        y = np.zeros(self.ndof)
+        L.check(L.load().tsem_space_slab_apply_host(self.h, L.dptr(yl), L.dptr(y)))
+        return y
+
     def dof_start(self):
         out = np.zeros(self.nv + self.ne + self.nf + self.nt, np.int64)
         L.check(L.load().tsem_space_dof_start(self.h, L.lptr(out)))

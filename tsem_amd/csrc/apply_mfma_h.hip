@@ -16,14 +16,19 @@
 //   U = D X (GEMM 1), V = Ghat U per element (FP32, into LDS), exact per-
 //   element max |V| (LDS atomics), V / max split to fp16 hi/lo in LDS
 //   (element-major, the B-operand layout), Y = D^T V (GEMM 2), Y rescaled
-//   and written element-major through an LDS transpose.
+//   and written element-major through an LDS transpose, or, from the same
+//   LDS tile, as entity-slab entries (the element's contribution to each of
+//   its global DOFs, written where the slab scatter reads it).
 //
 // MFMA 16x16x32 f16 operand layout (one 64-lane wave, measured on gfx950):
 // lane l supplies A[l % 16][8 (l / 16) + t] and B[8 (l / 16) + t][l % 16],
 // t = 0..7; accumulator value j of lane l is C[4 (l / 16) + j][l % 16].
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include "kernels.hpp"
+#include "slab.hpp"
 
 namespace tsem {
 
@@ -150,7 +155,7 @@ __global__ void __launch_bounds__(256) k_gather_tiles_h(DevSpace S, int Kl, cons
 template <int CT>
 __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, const uint16_t* __restrict__ xhi,
                                                     const uint16_t* __restrict__ xlo, const float* __restrict__ xs,
-                                                    float* __restrict__ yl) {
+                                                    float* __restrict__ yl, int slab) {
   constexpr int NE = 16 * CT;
   constexpr int VS = NE + 1;  // odd row stride of the FP32 LDS tiles: no bank conflicts
   extern __shared__ unsigned char smem_raw[];
@@ -322,6 +327,29 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
 
   // ---- phase D: rescale and write Y element-major through an LDS transpose
   float* Y = reinterpret_cast<float*>(smem_raw);  // [K][VS]
+  const int nval = min(NE, S.nt - e0);
+  // slab form: Y rows in slab order (sl_perm); the block's entity offsets
+  // and the transposed face rows are staged behind Y, in the dead fp16 V
+  // tile below the maxima (slab_h_fits), while Y is written, so that the
+  // epilogue reads LDS only
+  const int JT = S.sl_JT, nrow = S.sl_nrow;
+  int32_t* seo = nullptr;   // [NE][17] entity offsets, orientation words
+  float* stw = nullptr;     // [nrow][JT] weights
+  uint16_t* stk = nullptr;  // [nrow][JT] Y offsets of the terms' face modes (d VS)
+  uint16_t* stn = nullptr;  // [nrow] term counts
+  if (slab) {
+    const int nrj = nrow * JT;
+    seo = reinterpret_cast<int32_t*>(smem_raw + slab_tab_offset(K, NE));
+    stw = reinterpret_cast<float*>(seo + NE * 17);
+    stk = reinterpret_cast<uint16_t*>(stw + nrj);
+    stn = stk + nrj;
+    for (int s = tid; s < NE * 17; s += NT) seo[s] = (s < nval * 17) ? S.sl_eoff[(size_t)e0 * 17 + s] : 0;
+    for (int s = tid; s < nrj; s += NT) { stw[s] = S.sl_tw32[s]; stk[s] = (uint16_t)((S.sl_tk[s] & 255) * VS); }
+    for (int s = tid; s < nrow; s += NT) stn[s] = (uint16_t)(S.sl_tk[s * JT] >> 8);
+    int32_t* work = reinterpret_cast<int32_t*>(smem_raw + slab_tab_offset(K, NE) + slab_tab_bytes(S, NE));
+    if (tid < 2) work[tid] = 0;  // work-list counts
+    if (tid < 6) work[2 + tid] = (int32_t)S.sl_sgn[tid];
+  }
 #pragma unroll
   for (int c = 0; c < CT; ++c) {
     const int col = 16 * c + lr;
@@ -333,13 +361,125 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           int i = 16 * t2 + 4 * lk + q;
-          if (i < K) Y[(size_t)i * VS + col] = acc[j][c][q] * m;
+          if (i < K) Y[(size_t)(slab ? S.sl_perm[i] : i) * VS + col] = acc[j][c][q] * m;
         }
       }
     }
   }
   __syncthreads();
-  const int nval = min(NE, S.nt - e0);
+  if (slab) {  // full order only: every mode of Y is set
+    // The epilogue is bound by vector instructions (a wave64 instruction
+    // takes 4 cycles of a 16-lane SIMD), so it runs in two passes.  Pass A:
+    // every entry that is a copy -- vertices, interiors, DOFs of faces in a
+    // diagonal variant (+- the element's own face mode; the identity of
+    // variant 0 among them) and edges whose two faces are diagonal (own mode,
+    // edge sign) -- with one entry per thread, decoded once, walking the
+    // elements; consecutive threads hold consecutive DOFs of one entity, so
+    // stores come in runs.  Meanwhile the faces in other variants and the
+    // edges next to them go to LDS work lists (their order does not matter:
+    // every item writes its own entries); pass B sums their transposed rows
+    // term by term in table order (as slab_apply_host does on the host).
+    const int n1 = S.sl_n1, n2 = S.sl_n2, n3 = K - 4 - 6 * n1 - 4 * n2;
+    const int FB = 4 + 6 * n1;  // first face row of Y
+    const int dg = S.sl_diag;
+    int32_t* work = reinterpret_cast<int32_t*>(smem_raw + slab_tab_offset(K, NE) + slab_tab_bytes(S, NE));
+    const int32_t* sgn = work + 2;
+    int32_t* itemF = work + 8;        // [NE * 4] le * 4 + f
+    int32_t* itemE = itemF + NE * 4;  // [NE * 6] le * 6 + e
+    auto faces_of = [](int e, int& fa, int& fb) {  // the two faces holding edge e, lower first
+      const int a = e < 3 ? 0 : e < 5 ? 1 : 2, b = e < 3 ? e + 1 : e < 5 ? e - 1 : 3;
+      const int rest = 15 & ~((1 << a) | (1 << b));
+      fa = rest & 1 ? 0 : rest & 2 ? 1 : 2;
+      fb = rest & 8 ? 3 : rest & 4 ? 2 : 1;
+    };
+    if (tid < 4 * nval) {
+      yl[seo[(tid >> 2) * 17 + (tid & 3)]] = Y[(tid & 3) * VS + (tid >> 2)];
+      const int var = (seo[(tid >> 2) * 17 + 15] >> (6 * (tid & 3))) & 7;
+      if (!((dg >> var) & 1)) itemF[atomicAdd(&work[0], 1)] = tid;
+    }
+    for (int s = tid; s < 6 * nval; s += NT) {
+      const int le = s / 6, e = s - 6 * le;
+      int fa, fb;
+      faces_of(e, fa, fb);
+      const int32_t ow = seo[le * 17 + 15];
+      if (!((dg >> ((ow >> (6 * fa)) & 7)) & (dg >> ((ow >> (6 * fb)) & 7)) & 1)) itemE[atomicAdd(&work[1], 1)] = s;
+    }
+    if (n1 > 0) {
+      const int EE = 6 * n1, G = NT / EE, g = tid / EE, r = tid - g * EE;
+      if (g < G) {
+        const int e = r / n1, l = r - e * n1;
+        int fa, fb;
+        faces_of(e, fa, fb);
+        const int yo = (4 + e * n1 + l) * VS;
+        for (int le = g; le < nval; le += G) {
+          const int32_t* eo = seo + le * 17;
+          const int32_t ow = eo[15];
+          if (!((dg >> ((ow >> (6 * fa)) & 7)) & (dg >> ((ow >> (6 * fb)) & 7)) & 1)) continue;
+          const float v = Y[yo + le];
+          yl[eo[4 + e] + l] = (((ow >> (24 + e)) & 1) && (l & 1)) ? -v : v;
+        }
+      }
+    }
+    if (n2 > 0) {
+      const int EF = 4 * n2, G = NT / EF, g = tid / EF, r = tid - g * EF;
+      if (g < G) {
+        const int f = r / n2, i = r - f * n2;
+        const int yo = (FB + f * n2 + i) * VS;
+        for (int le = g; le < nval; le += G) {
+          const int32_t* eo = seo + le * 17;
+          const int var = (eo[15] >> (6 * f)) & 7;
+          if (!((dg >> var) & 1)) continue;
+          const float v = Y[yo + le];
+          yl[eo[10 + f] + i] = (i < 32 && ((sgn[var] >> i) & 1)) ? -v : v;
+        }
+      }
+    }
+    if (n3 > 0) {
+      const int G = NT / n3, g = tid / n3, i = tid - g * n3;
+      if (g < G)
+        for (int le = g; le < nval; le += G) yl[seo[le * 17 + 14] + i] = Y[(K - n3 + i) * VS + le];
+    }
+    __syncthreads();
+    // pass B: faces in a non-diagonal variant, then the edges next to them
+    const int nF = work[0], nE = work[1];
+    for (int s = tid; s < nF * n2; s += NT) {
+      const int it = s / n2, i = s - it * n2;
+      const int le = itemF[it] >> 2, f = itemF[it] & 3;
+      const int32_t* eo = seo + le * 17;
+      const int row = ((eo[15] >> (6 * f)) & 7) * n2 + i;
+      const float* w = stw + row * JT;
+      const uint16_t* k = stk + row * JT;
+      const float* Yf = Y + (FB + f * n2) * VS + le;
+      float v = 0.0f;
+      for (int t = 0; t < stn[row]; ++t) v += w[t] * Yf[k[t]];
+      yl[eo[10 + f] + i] = v;
+    }
+    for (int s = tid; s < nE * n1; s += NT) {
+      const int it = s / n1, l = s - it * n1;
+      const int le = itemE[it] / 6, e = itemE[it] - 6 * le;
+      const int32_t* eo = seo + le * 17;
+      const int32_t ow = eo[15], ow2 = eo[16];
+      int fh[2];
+      faces_of(e, fh[0], fh[1]);
+      float v = Y[(4 + e * n1 + l) * VS + le];
+      if (((ow >> (24 + e)) & 1) && (l & 1)) v = -v;
+      float part[2] = {0.0f, 0.0f};
+      for (int h = 0; h < 2; ++h) {
+        const int f = fh[h], var = (ow >> (6 * f)) & 7;
+        if ((dg >> var) & 1) continue;
+        const int q = (ow2 >> (4 * e + 2 * h)) & 3;
+        const int row = 6 * n2 + (var * 3 + q) * n1 + l;
+        const float* w = stw + row * JT;
+        const uint16_t* k = stk + row * JT;
+        const float* Yf = Y + (FB + f * n2) * VS + le;
+        float p = 0.0f;
+        for (int t = 0; t < stn[row]; ++t) p += w[t] * Yf[k[t]];
+        part[h] = (((ow >> (6 * f + 3 + q)) & 1) && (l & 1)) ? -p : p;
+      }
+      yl[eo[4 + e] + l] = (v + part[0]) + part[1];
+    }
+    return;
+  }
   for (int s = tid; s < nval * K; s += NT) {
     int le = s / K, i = s % K;
     yl[(size_t)(e0 + le) * K + i] = (i < Kl && i < 16 * Rt) ? Y[(size_t)i * VS + le] : 0.0f;
@@ -359,6 +499,23 @@ size_t gemm_h_colmax_offset(const DevSpace& S) {
   return m > y ? m : y;
 }
 
+// end of the slab data staged behind the Y tile (slab_tab_offset): the
+// tables (slab_tab_bytes), then the epilogue's work-list counts, sign masks
+// and work lists (8 + NE * 10 ints)
+static size_t slab_tab_end(const DevSpace& S) {
+  const int NE = 16 * gemm_h_ct(S);
+  return slab_tab_offset(S.K, NE) + slab_tab_bytes(S, NE) + (size_t)(8 + NE * 10) * 4;
+}
+
+// the tables fit below the maxima, and each entity kind's DOFs of one
+// element fit in one block of threads (the epilogue gives each thread one)
+bool slab_h_fits(const DevSpace& S) {
+  const int n3 = S.K - 4 - 6 * S.sl_n1 - 4 * S.sl_n2;
+  const int VS = 16 * gemm_h_ct(S) + 1;
+  return slab_tab_end(S) <= S.hlds_v && 6 * S.sl_n1 <= NT && 4 * S.sl_n2 <= NT && n3 <= NT &&
+         (S.sl_n2 - 1) * VS < 65536;
+}
+
 size_t gemm_h_lds(const DevSpace& S) { return gemm_h_colmax_offset(S) + (size_t)16 * gemm_h_ct(S) * 4; }
 
 bool mfma_h_supported(const DevSpace& S) {
@@ -367,7 +524,8 @@ bool mfma_h_supported(const DevSpace& S) {
          3 * (S.Mh / 8) * NE <= GPT * NT && gemm_h_lds(S) <= 160 * 1024;
 }
 
-void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, float* yl, hipStream_t st) {
+void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, float* yl, bool slab, hipStream_t st) {
+  if (slab && (op.Kl != 0 || !S.sl_meta)) throw std::logic_error("slab results need the full order and slab tables");
   const int Kl = op.Kl > 0 ? op.Kl : S.K;
   const bool timed = op.Kl == 0;
   int th = timed ? timing_begin(TK_GATHER32, st) : -1;
@@ -380,9 +538,11 @@ void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, fl
   int nb = (S.nt + NE - 1) / NE;
   th = timed ? timing_begin(TK_ELEM32, st) : -1;
   if (CT == 4)
-    hipLaunchKernelGGL(k_elem_gemm_h<4>, dim3(nb), dim3(NT), gemm_h_lds(S), st, S, op, S.xhhi, S.xhlo, S.xhs, yl);
+    hipLaunchKernelGGL(k_elem_gemm_h<4>, dim3(nb), dim3(NT), gemm_h_lds(S), st, S, op, S.xhhi, S.xhlo, S.xhs, yl,
+                       (int)slab);
   else
-    hipLaunchKernelGGL(k_elem_gemm_h<2>, dim3(nb), dim3(NT), gemm_h_lds(S), st, S, op, S.xhhi, S.xhlo, S.xhs, yl);
+    hipLaunchKernelGGL(k_elem_gemm_h<2>, dim3(nb), dim3(NT), gemm_h_lds(S), st, S, op, S.xhhi, S.xhlo, S.xhs, yl,
+                       (int)slab);
   timing_end(th, st);
 }
 

@@ -244,6 +244,7 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
         sp->w3[k] = (double)(6 * (1 - t[j]) * (1 - t[l]) * (1 - t[l]) * w[i] * w[j] * w[l]);
       }
   sp->cmask.assign(S.ndof, 0);
+  tsem::build_slab(S, sp->slab);
   if (!ctx) {  // host-only space: setup and map export without a device
     S.diagonal(1.0, 0.0, sp->diag);
     *out = sp.release();
@@ -434,6 +435,18 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   {
     d.hlds_v = gemm_h_colmax_offset(d);
   }
+  if (sp->slab.ok) {
+    const tsem::Slab& L = sp->slab;
+    upload(sp->sl_meta, L.meta, st); upload(sp->sl_chunk, L.chunk, st); upload(sp->sl_eoff, L.eoff, st);
+    upload(sp->sl_perm, L.perm, st); upload(sp->sl_tk, L.tk, st); upload(sp->sl_tw, L.tw, st);
+    upload(sp->sl_tw32, std::vector<float>(L.tw.begin(), L.tw.end()), st);
+    d.sl_n1 = L.n1; d.sl_n2 = L.n2; d.sl_JT = L.JT; d.sl_nrow = L.nrow; d.sl_chunk_max = L.chunk_max;
+    d.sl_diag = L.diag;
+    for (int v = 0; v < 6; ++v) d.sl_sgn[v] = L.sgn[v];
+    d.sl_meta = sp->sl_meta.as<int32_t>(); d.sl_chunk = sp->sl_chunk.as<int32_t>();
+    d.sl_eoff = sp->sl_eoff.as<int32_t>(); d.sl_perm = sp->sl_perm.as<int32_t>();
+    d.sl_tk = sp->sl_tk.as<uint16_t>(); d.sl_tw = sp->sl_tw.as<double>(); d.sl_tw32 = sp->sl_tw32.as<float>();
+  }
   // assembled diagonal on the device from the scatter map, host copy for
   // the boundary-condition setup
   sp->diag_d.alloc((size_t)S.ndof * 8);
@@ -515,6 +528,25 @@ int tsem_space_maps(const tsem_space* sp, int32_t* off, int32_t* ind, double* va
       for (auto& pr : tm) { ind[n] = (int32_t)pr.first; val[n] = pr.second; ++n; }
       off[(size_t)e * S.K + i + 1] = (int32_t)n;
     }
+  API_END
+}
+
+int tsem_space_slab(const tsem_space* sp, int64_t* out) {
+  API_BEGIN
+  if (!sp || !out) throw std::invalid_argument("null argument");
+  const DevSpace& d = sp->d;
+  out[0] = (d.sl_meta && slab_enabled() && mfma_h_supported(d) && slab_h_fits(d) && slab_scatter_fits(d) &&
+            apply_path() == 0) ? 1 : 0;
+  out[1] = sp->slab.ok ? sp->slab.total : 0;
+  out[2] = sp->slab.chunk_max;
+  out[3] = sp->slab.JT;
+  API_END
+}
+
+int tsem_space_slab_apply_host(const tsem_space* sp, const double* yl, double* y) {
+  API_BEGIN
+  if (!sp || !yl || !y) throw std::invalid_argument("null argument");
+  tsem::slab_apply_host(sp->h, sp->slab, yl, y);
   API_END
 }
 
@@ -937,6 +969,12 @@ int tsem_set_graphs(int enable) {
   API_END
 }
 
+int tsem_set_slab(int enable) {
+  API_BEGIN
+  set_slab(enable != 0);
+  API_END
+}
+
 int tsem_set_apply_path(int path) {
   API_BEGIN
   if (path < 0 || path > 2) throw std::invalid_argument("path must be 0 (auto), 1 (scalar) or 2 (no fp16 split)");
@@ -1032,7 +1070,8 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
       // the chunk of nrun iterations as one captured graph, replayed while
       // the pointers and parameters it captured stay the same
       key = {x, r, z, p, q, yl, part, sc, sp->pmg.get(), (const void*)(intptr_t)precond,
-             (const void*)(intptr_t)smooth, (const void*)(intptr_t)nrun, (const void*)(intptr_t)apply_path()};
+             (const void*)(intptr_t)smooth, (const void*)(intptr_t)nrun, (const void*)(intptr_t)apply_path(),
+             (const void*)(intptr_t)slab_enabled()};
       if ((!G.exec || G.key != key) && G.failed != key) {
         // a capture can be invalidated from outside (another thread's
         // device-wide synchronization): its launches never ran, so the chunk

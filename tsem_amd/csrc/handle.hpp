@@ -12,6 +12,7 @@
 
 #include "kernels.hpp"
 #include "pmg.hpp"
+#include "slab.hpp"
 #include "space.hpp"
 
 
@@ -84,6 +85,8 @@ struct tsem_space {
   // quadrature-point work: basis at the points, weights, potential, scratch
   DevBuf qphi, qphiT, qdphi, qw, vpot, qxl, qpart, qout, grad2;
   DevBuf Dh1hi, Dh1lo, Dh2hi, Dh2lo, xhhi, xhlo, xhs;  // fp16-split FP32 operator
+  tsem::Slab slab;                                      // entity-slab layout (host tables)
+  DevBuf sl_meta, sl_chunk, sl_eoff, sl_perm, sl_tk, sl_tw, sl_tw32;
   // solver vectors
   DevBuf v64[6], v32[6];
   // instantiated graph of one chunk of PCG iterations per precision, valid

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "slab.hpp"
 
 namespace tsem {
 
@@ -293,6 +294,7 @@ __global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict
 // is that the loads no longer wait on each other row by row.  Chunks are
 // spread over XCDs like k_scatter's rows (XCD x owns a contiguous range).
 constexpr int SCH = BLK;
+static_assert(SCH == SLAB_ROWS, "one slab chunk per scatter block");
 template <typename T>
 __global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __restrict__ yl, T* __restrict__ y,
                                                         const T* __restrict__ xin, int constrained,
@@ -345,6 +347,94 @@ __global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __r
   if (!partial) return;
   double tot = block_sum(d, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// Scatter of entity-slab results (slab.hpp): chunk c of 256 DOFs reads its
+// slab span [sl_chunk[2c], + sl_chunk[2c + 1]) into LDS with coalesced
+// loads; DOF g then adds its m entries, spaced by its geometry's DOF count,
+// in element order.  The slab already carries the signs and face weights,
+// so there are no codes, weights or indirect loads.  A block takes SP
+// consecutive chunks at a time and issues all their loads before waiting
+// (the loads in flight, not the instructions, bound this kernel).
+// Eliminated rows, partial sums and the XCD chunk ranges are those of
+// k_scatter_chunks.
+constexpr int SP = 4;   // chunks per step
+constexpr int SU = 4;   // span entries per thread and chunk: spans <= SU * BLK
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_scatter_slab(DevSpace S, const T* __restrict__ sl, T* __restrict__ y,
+                                                      const T* __restrict__ xin, int constrained,
+                                                      double* __restrict__ partial) {
+  extern __shared__ unsigned char scat_raw[];
+  T* vals = reinterpret_cast<T*>(scat_raw);  // [SP][sl_chunk_max]
+  __shared__ double red[BLK];
+  double d = 0.0;
+  const int nch = (S.ndof + SCH - 1) / SCH;
+  const int ngr = (nch + SP - 1) / SP;
+  const int nx = min((int)gridDim.x, NXCD);
+  const int xcd = blockIdx.x % NXCD, jb = blockIdx.x / NXCD;
+  const int nbx = gridDim.x / NXCD + (xcd < (int)(gridDim.x % NXCD) ? 1 : 0);
+  const int q0 = (int)((long long)xcd * ngr / nx), q1 = (int)((long long)(xcd + 1) * ngr / nx);
+  const int n1 = S.sl_n1, n2 = S.sl_n2, cm = S.sl_chunk_max;
+  for (int q = q0 + jb; q < q1; q += nbx) {
+    int meta[SP];
+    T v[SP][SU];
+#pragma unroll
+    for (int p = 0; p < SP; ++p) {
+      const int c = q * SP + p;
+      const int g = c * SCH + (int)threadIdx.x;
+      meta[p] = (c < nch && g < S.ndof) ? S.sl_meta[g] : 0;
+      const int lo = c < nch ? S.sl_chunk[2 * c] : 0, len = c < nch ? S.sl_chunk[2 * c + 1] : 0;
+#pragma unroll
+      for (int u = 0; u < SU; ++u) {
+        const int t = (int)threadIdx.x + u * BLK;
+        v[p][u] = (t < len) ? sl[lo + t] : T(0);
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < SP; ++p)
+#pragma unroll
+      for (int u = 0; u < SU; ++u) {
+        const int t = (int)threadIdx.x + u * BLK;
+        if (t < cm) vals[p * cm + t] = v[p][u];
+      }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < SP; ++p) {
+      const int g = (q * SP + p) * SCH + (int)threadIdx.x;
+      if (g < S.ndof) {
+        const int rel = meta[p] & 0xfffff, m = (meta[p] >> 20) & 0x3ff, dim = (int)((unsigned)meta[p] >> 30);
+        const int stride = dim == 1 ? n1 : dim == 2 ? n2 : 1;
+        const T* row = vals + p * cm + rel;
+        T acc = T(0);
+        for (int k = 0; k < m; ++k) acc += row[k * stride];
+        if (constrained && S.cmask[g]) acc = (T)S.diag[g] * xin[g];
+        y[g] = acc;
+        if (partial) d += (double)xin[g] * (double)acc;
+      }
+    }
+    __syncthreads();  // vals is refilled by the next step
+  }
+  if (!partial) return;
+  double tot = block_sum(d, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// every chunk span is loaded by one pass of SU loads per thread
+bool slab_scatter_fits(const DevSpace& S) { return S.sl_chunk_max <= SU * BLK; }
+
+static bool slab_from_env() {
+  const char* v = getenv("TSEM_SLAB");
+  return !(v && v[0] == '0');
+}
+static bool g_slab = slab_from_env();
+void set_slab(bool on) { g_slab = on; }
+bool slab_enabled() { return g_slab; }
+
+template <typename T>
+static void slab_scatter(const DevSpace& S, const T* sl, T* y, const T* x, int constrained, double* partial,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_scatter_slab<T>, dim3(reduce_blocks(S.ndof)), dim3(BLK),
+                     (size_t)SP * S.sl_chunk_max * sizeof(T), st, S, sl, y, x, constrained, partial);
 }
 
 // the chunked scatter when a chunk's terms fit in LDS, else the row kernel
@@ -415,8 +505,12 @@ int apply_path() { return g_apply_path; }
 template <typename T>
 static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, double* partial, hipStream_t st) {
   const int kind = (sizeof(T) == 8) ? TK_GATHER64 : TK_GATHER32;
+  bool slab = false;
   if (sizeof(T) == 4 && op.mass == 0.0 && g_apply_path == 0 && mfma_h_supported(S)) {
-    launch_apply_mfma_h(S, op, (const float*)x, (float*)yl, st);
+    // entity-slab results when the element kernel's output goes straight to
+    // the scatter (full order, no potential term added to it)
+    slab = g_slab && S.sl_meta && op.Kl == 0 && op.pot == 0.0 && slab_scatter_fits(S) && slab_h_fits(S);
+    launch_apply_mfma_h(S, op, (const float*)x, (float*)yl, slab, st);
   } else if (op.mass == 0.0 && (g_apply_path == 0 || g_apply_path == 2) && mfma_supported(S)) {
     launch_apply_mfma<T>(S, op, x, yl, st);
   } else {
@@ -434,7 +528,8 @@ static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T*
     hipLaunchKernelGGL(k_apply_potential<T>, dim3(S.nt), dim3(BLK), lds, st, S, op.pot, x, yl, op.constrained);
   }
   int th = (op.Kl == 0) ? timing_begin(kind + 2, st) : -1;
-  scatter_dispatch<T>(S, yl, y, x, op.constrained, partial, st);
+  if (slab) slab_scatter<T>(S, yl, y, x, op.constrained, partial, st);
+  else scatter_dispatch<T>(S, yl, y, x, op.constrained, partial, st);
   timing_end(th, st);
 }
 

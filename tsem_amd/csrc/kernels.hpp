@@ -101,6 +101,21 @@ struct DevSpace {
   // tiled scratch for gathered element vectors
   double* xt64 = nullptr;
   float* xt32 = nullptr;
+  // entity-slab layout of element results (slab.hpp); present when sl_meta
+  // is set: per DOF meta (offset in its chunk | count << 20 | dim << 30),
+  // per chunk of 256 DOFs its span (start, length), per element 15 entity
+  // offsets and two orientation words, local mode -> slab row, transposed
+  // face rows (slab.hpp)
+  int sl_n1 = 0, sl_n2 = 0, sl_JT = 1, sl_nrow = 1, sl_chunk_max = 0;
+  int sl_diag = 1;                    // diagonal face variants (bit v)
+  uint32_t sl_sgn[6] = {0, 0, 0, 0, 0, 0};  // their minus signs per face DOF
+  const int32_t* sl_meta = nullptr;
+  const int32_t* sl_chunk = nullptr;
+  const int32_t* sl_eoff = nullptr;  // [nt][17]
+  const int32_t* sl_perm = nullptr;  // [K] slab row of local mode i
+  const uint16_t* sl_tk = nullptr;
+  const double* sl_tw = nullptr;
+  const float* sl_tw32 = nullptr;
 };
 
 // operator selection
@@ -123,11 +138,28 @@ void launch_apply(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, 
 template <typename T>
 void launch_apply_mfma(const DevSpace& S, const OpCoef& op, const T* x, T* yl, hipStream_t st);
 bool mfma_supported(const DevSpace& S);
-// fp16-split path for FP32 vectors (x: FP32 DOF vector, yl: element results)
-void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, float* yl, hipStream_t st);
+// fp16-split path for FP32 vectors (x: FP32 DOF vector, yl: element results,
+// element-major, or the entity slab when slab is set)
+void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, float* yl, bool slab, hipStream_t st);
+// entity-slab element results on (process wide; default on, TSEM_SLAB=0 off)
+void set_slab(bool on);
+bool slab_enabled();
 bool mfma_h_supported(const DevSpace& S);
 size_t gemm_h_lds(const DevSpace& S);
 size_t gemm_h_colmax_offset(const DevSpace& S);  // DevSpace::hlds_v
+// LDS byte offset of the slab tables behind the [K][NE + 1] FP32 Y tile
+__host__ __device__ inline size_t slab_tab_offset(int K, int NE) { return ((size_t)K * (NE + 1) * 4 + 15) & ~(size_t)15; }
+// bytes of the slab tables staged there: entity offsets and orientation
+// words [NE][17], FP32 weights [nrow][JT], 16-bit Y offsets [nrow][JT],
+// 16-bit term counts [nrow], rounded up to 4 bytes
+__host__ __device__ inline size_t slab_tab_bytes(const DevSpace& S, int NE) {
+  const size_t nrj = (size_t)S.sl_nrow * S.sl_JT;
+  return ((size_t)NE * 68 + nrj * 6 + (size_t)S.sl_nrow * 2 + 3) & ~(size_t)3;
+}
+// the slab tables fit between the Y tile and the maxima
+bool slab_h_fits(const DevSpace& S);
+// the slab scatter takes every chunk span in one pass
+bool slab_scatter_fits(const DevSpace& S);
 // per-kernel HIP-event timing of full-order applies (kinds below); timing_read
 // fills {total ms, launches} per kind
 enum { TK_GATHER64 = 0, TK_ELEM64, TK_SCATTER64, TK_GATHER32, TK_ELEM32, TK_SCATTER32, TK_KINDS };

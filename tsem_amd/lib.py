@@ -75,6 +75,9 @@ SIGNATURES = {
     "tsem_scatter": [_vp, _vp, _vp],
     "tsem_diagonal": [_vp, _vp],
     "tsem_set_apply_path": [C.c_int],
+    "tsem_set_slab": [C.c_int],
+    "tsem_space_slab_apply_host": [_vp, _vp, _vp],
+    "tsem_space_slab": [_vp, _vp],
     "tsem_set_graphs": [C.c_int],
     "tsem_timing": [C.c_int],
     "tsem_timing_read": [_dp],
