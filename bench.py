@@ -139,6 +139,20 @@ def serve_section(log, count=24, slots=8, seed=2026, loads=(0.8, 0.5)):
         by_load[str(load)] = {"arrival_rate_per_s": rt, "dofs_per_s": stm["dofs_per_s"], "p50_ms": stm["p50_ms"],
                               "p99_ms": stm["p99_ms"], "latency_over_isolated_p50": float(np.percentile(rr, 50)),
                               "latency_over_isolated_p99": float(np.percentile(rr, 99))}
+    # the same stream at the headline load on the C ABI server (workers,
+    # lanes, pool and classes in the library; the meshes are the caller's, made
+    # before the clock starts; small = below the median element work)
+    from math import comb
+    small = float(np.median([6 * n ** 3 * comb(p + 3, 3) for n in range(8, 33) for p in range(3, 9)]))
+    nat = serve.NativeServer(slots=slots, small_cost=small)
+    nat.run(serve.make_requests(count, seed=seed))
+    nreqs, nstream = nat.run(serve.make_requests(count, seed=seed, rate=rate))
+    nat.close()
+    nratio = np.array([(r.done - r.arrival) / i for r, i in zip(nreqs, iso)])
+    native = {"p50_ms": nstream["p50_ms"], "p99_ms": nstream["p99_ms"], "dofs_per_s": nstream["dofs_per_s"],
+              "max_relres": nstream["max_relres"],
+              "latency_over_isolated": {"p50": float(np.percentile(nratio, 50)),
+                                        "p99": float(np.percentile(nratio, 99))}}
     for k in np.argsort(ratio)[::-1][:4]:
         r = reqs[k]
         log("serving worst: n %d p %d dofs %d wait %.3f s service %.3f s isolated %.3f s built %s worker %d"
@@ -152,7 +166,7 @@ def serve_section(log, count=24, slots=8, seed=2026, loads=(0.8, 0.5)):
                latency_over_isolated={"p50": float(np.percentile(ratio, 50)), "p99": float(np.percentile(ratio, 99)),
                                       "max": float(ratio.max())},
                isolated_ms={"p50": float(np.percentile(iso, 50)) * 1e3, "p99": float(np.percentile(iso, 99)) * 1e3},
-               by_load=by_load)
+               by_load=by_load, native_c_abi=native)
     log("serving %s" % out)
     return out
 

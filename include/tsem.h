@@ -332,6 +332,82 @@ int tsem_set_potential(tsem_space* sp, const double* v_dev);
 int tsem_apply_potential(tsem_space* sp, double stiff, double mass, double potential, int constrained,
                          const double* x_dev, double* y_dev);
 
+/* ---- served stream of Poisson requests -------------------------------
+ * The reference has no served form; these four entry points are the
+ * boundary of a service that solves many independent Poisson problems on
+ * one device.  A request is the mixed-BC problem of
+ * examples/poisson_mixBC/poisson.cpp:74-89 (load vector, Neumann load on
+ * mark 2, Dirichlet elimination on mark 1), solved by PCG to a relative
+ * residual.  Workers (one context each) take the earliest request of their
+ * lane -- the first small_slots workers only small requests, the others the
+ * earliest large one, else the earliest small one -- as soon as its device
+ * footprint fits the budget; spaces are cached per topology key and only
+ * their coordinates replaced; with classes, small requests run foreground
+ * and large ones background (tsem_ctx_set_class). */
+typedef struct tsem_server tsem_server;
+
+/* f at n device points (xyz_dev: n x 3) into out_dev, enqueued on `stream`
+ * (a hipStream_t) or finished before returning; nonzero = failure.  Called
+ * from a worker thread. */
+typedef int (*tsem_load_fn)(void* user, const double* xyz_dev, int64_t n, double* out_dev, void* stream);
+/* u at n host points (n x 3) into out (host) */
+typedef int (*tsem_value_fn)(void* user, const double* xyz, int64_t n, double* out);
+/* du/dn at n host points with the outward unit normals (n x 3) into out */
+typedef int (*tsem_flux_fn)(void* user, const double* xyz, const double* normal, int64_t n, double* out);
+
+typedef struct {
+  int slots;          /* worker threads, one context (stream) each; 0: 8 */
+  int small_slots;    /* workers that take only small requests; -1: slots / 2 */
+  double small_cost;  /* a request is small when n_elem * C(M+3,3) <= small_cost; 0: 4e6 */
+  int64_t hbm_budget; /* device bytes the spaces in use and cached may hold; 0: 70% of free */
+  int classes;        /* small requests foreground, large ones background */
+} tsem_serve_opts;
+
+typedef struct {
+  int64_t topology;       /* caller's key: requests with one nonzero key share connectivity, order,
+                             quadrature and boundary labels, and reuse a cached space; 0: none */
+  int M, qorder;          /* order; quadrature order (0: M + 3) */
+  int nv, nt;
+  const double* xyz;      /* host, nv x 3 (copied by submit) */
+  const int32_t* tet;     /* host, nt x 4 (copied by submit) */
+  const int32_t* labels;  /* host, nv: bits of the boundary parts each vertex lies on (copied); NULL: none */
+  int32_t dirichlet;      /* label bits of the Dirichlet parts (mark 1); a geometry lies on a part when
+                             all its vertices do */
+  int32_t neumann;        /* label bits of the Neumann parts (mark 2); Dirichlet wins */
+  tsem_load_fn f;         /* required */
+  tsem_value_fn u;        /* required when a Dirichlet part exists */
+  tsem_flux_fn g;         /* required when a Neumann part exists */
+  void* user;             /* passed to the callbacks */
+  double tol;             /* relative residual ||b - Ax||_2 / ||b||_2 (0: 1e-10) */
+  int mixed;              /* tsem_pcg_opts.mixed (inner_tol 1e-2) */
+  int precond;            /* 1 Jacobi (0 means 1), 4 p-multigrid */
+} tsem_serve_request;
+
+typedef struct {
+  int status;             /* TSEM_OK, TSEM_ENOCONV, or the request's error */
+  int built;              /* 1 when the request built its space (no cached one) */
+  int64_t n_dof;
+  int iterations, outer;
+  double relres;
+  double queued_s;         /* submit to start */
+  double space_s;          /* space built, or its coordinates replaced */
+  double setup_s;          /* load vector and boundary data */
+  double solve_s;          /* PCG and the copy of the solution */
+  double total_s;          /* submit to completion */
+} tsem_serve_result;
+
+int tsem_serve_open(int device, const tsem_serve_opts* opts, tsem_server** out);
+/* Queues a request; *id names it in tsem_serve_poll. */
+int tsem_serve_submit(tsem_server* s, const tsem_serve_request* req, int64_t* id);
+/* *done = 0 while request id runs (wait != 0 blocks until it has finished).
+ * Once finished: *done = 1, res filled (n_dof included) and the request's
+ * status returned (its message in tsem_last_error); with x (host, n_dof
+ * doubles) the solution is copied there and the request released, with x
+ * NULL it is kept for a later poll. */
+int tsem_serve_poll(tsem_server* s, int64_t id, int wait, double* x, tsem_serve_result* res, int* done);
+/* Solves every queued request, then frees the server (results not polled are dropped). */
+int tsem_serve_close(tsem_server* s);
+
 #ifdef __cplusplus
 }
 #endif

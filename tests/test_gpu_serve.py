@@ -64,3 +64,82 @@ def test_foreground_and_background_classes(ctx):
         (one,), _ = alone.run([r])
         assert np.array_equal(one.solution, got[r.id].solution), r
     alone.close()
+
+
+# ------------------------------------------------ C ABI served stream ----
+def _python_solution(ctx, n, p, seed):
+    """The request solved on its own through the Python path (poisson_setup
+    and the mixed PCG the Python server runs)."""
+    from tsem_amd import api, mesh, problems
+    xyz, tet = mesh.kuhn_cube(n, jitter=0.1, seed=seed)
+    sp = api.Space(ctx, p, xyz, tet, qorder=p + 3)
+    b = api.poisson_setup(sp, problems.Manufactured(seed))
+    x, st = sp.pcg(b, tol=1e-10, criterion=1, precond=1, mixed=True, inner_tol=1e-2, check_every=10,
+                   max_iter=100000)
+    out = x.cpu().numpy()
+    sp.close()
+    return out, st
+
+
+def _submit(srv, n, p, seed, topology=0):
+    from tsem_amd import mesh, problems
+    xyz, tet = mesh.kuhn_cube(n, jitter=0.1, seed=seed)
+    return srv.submit(xyz, tet, p, problems.Manufactured(seed), labels=mesh.cube_plane_marks(xyz),
+                      topology=topology)
+
+
+def test_native_server_matches_python_path(ctx):
+    """tsem_serve_*: requests of several sizes in flight together on four
+    workers give the Python path's solutions.  Load and boundary data are
+    assembled by other code paths (C++ normals, chunked load), so the two
+    solves agree to their tolerance (relative residual 1e-10; the solutions
+    to 1e-9), not bit for bit."""
+    _say("native serve mix")
+    reqs = [(4, 3, 1), (6, 4, 2), (3, 5, 3), (5, 6, 4), (7, 3, 5), (4, 4, 6)]
+    srv = serve.NativeServer(slots=4, small_cost=3e4)
+    ids = [_submit(srv, n, p, seed) for n, p, seed in reqs]
+    got = [srv.poll(i) for i in ids]
+    srv.close()
+    for (n, p, seed), (done, x, r) in zip(reqs, got):
+        ref, st = _python_solution(ctx, n, p, seed)
+        assert done and r["status"] == 0 and r["built"] == 1
+        assert r["n_dof"] == len(ref) and r["relres"] <= 1e-10, r
+        assert np.abs(x - ref).max() <= 1e-9 * np.abs(ref).max(), (n, p, seed)
+
+
+def test_native_server_reuses_cached_space(ctx):
+    """A request with the topology key of a finished one reuses its space
+    (only the coordinates change) and still matches the Python path."""
+    _say("native serve reuse")
+    srv = serve.NativeServer(slots=2)
+    a = _submit(srv, 4, 4, 11, topology=7)
+    _, xa, ra = srv.poll(a)
+    b = _submit(srv, 4, 4, 12, topology=7)
+    _, xb, rb = srv.poll(b)
+    srv.close()
+    assert ra["built"] == 1 and rb["built"] == 0
+    for x, seed in ((xa, 11), (xb, 12)):
+        ref, _ = _python_solution(ctx, 4, 4, seed)
+        assert np.abs(x - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def test_native_server_reports_callback_failure(ctx):
+    """A load callback that fails ends that request with TSEM_EINVAL and its
+    message; the server goes on with the next request."""
+    from tsem_amd import lib, mesh, problems
+
+    class Broken(problems.Manufactured):
+        def f(self, p):
+            raise RuntimeError("no load")
+
+    _say("native serve error")
+    srv = serve.NativeServer(slots=1)
+    xyz, tet = mesh.kuhn_cube(3)
+    bad = srv.submit(xyz, tet, 3, Broken(1), labels=mesh.cube_plane_marks(xyz))
+    good = _submit(srv, 3, 3, 2)
+    with pytest.raises(lib.TsemError) as e:
+        srv.poll(bad)
+    assert e.value.code == 2 and "load callback" in str(e.value)
+    done, x, r = srv.poll(good)
+    srv.close()
+    assert done and r["status"] == 0 and r["relres"] <= 1e-10

@@ -78,6 +78,10 @@ SIGNATURES = {
     "tsem_set_slab": [C.c_int],
     "tsem_space_slab_apply_host": [_vp, _vp, _vp],
     "tsem_space_slab": [_vp, _vp],
+    "tsem_serve_open": [C.c_int, _vp, _vp],
+    "tsem_serve_submit": [_vp, _vp, _vp],
+    "tsem_serve_poll": [_vp, C.c_int64, C.c_int, _vp, _vp, _vp],
+    "tsem_serve_close": [_vp],
     "tsem_set_graphs": [C.c_int],
     "tsem_timing": [C.c_int],
     "tsem_timing_read": [_dp],

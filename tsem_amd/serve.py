@@ -268,3 +268,154 @@ class Server:
                         ctx.close()
             self.pool.clear()
             self.in_use = 0
+
+
+# ---------------------------------------------------------------- C ABI ----
+import ctypes as C  # noqa: E402
+
+from . import lib as L  # noqa: E402
+
+_LOAD = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p)
+_VALUE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_double))
+_FLUX = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64,
+                    C.POINTER(C.c_double))
+
+
+class ServeOpts(C.Structure):
+    _fields_ = [("slots", C.c_int), ("small_slots", C.c_int), ("small_cost", C.c_double),
+                ("hbm_budget", C.c_int64), ("classes", C.c_int)]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("topology", C.c_int64), ("M", C.c_int), ("qorder", C.c_int), ("nv", C.c_int), ("nt", C.c_int),
+                ("xyz", C.c_void_p), ("tet", C.c_void_p), ("labels", C.c_void_p),
+                ("dirichlet", C.c_int32), ("neumann", C.c_int32),
+                ("f", _LOAD), ("u", _VALUE), ("g", _FLUX), ("user", C.c_void_p),
+                ("tol", C.c_double), ("mixed", C.c_int), ("precond", C.c_int)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("built", C.c_int), ("n_dof", C.c_int64), ("iterations", C.c_int),
+                ("outer", C.c_int), ("relres", C.c_double), ("queued_s", C.c_double), ("space_s", C.c_double),
+                ("setup_s", C.c_double), ("solve_s", C.c_double), ("total_s", C.c_double)]
+
+
+class _DeviceArray:
+    """A device buffer of doubles seen by torch (__cuda_array_interface__)."""
+
+    def __init__(self, ptr, shape):
+        self.__cuda_array_interface__ = {"shape": shape, "typestr": "<f8", "data": (ptr, False), "version": 3,
+                                         "strides": None}
+
+
+class NativeServer:
+    """The served stream of the C ABI (tsem_serve_open / submit / poll /
+    close): worker threads, lanes, space pool and solve classes in the
+    library; problem data through callbacks.  `problem` objects give f, u and
+    du/dn as torch functions of points (problems.Manufactured): f runs on the
+    library's stream for the request, u and du/dn on the device and come back
+    to the host."""
+
+    def __init__(self, device=0, slots=8, small_slots=-1, small_cost=0.0, hbm_budget=0, classes=True):
+        self.device = device
+        o = ServeOpts(slots, small_slots, small_cost, hbm_budget, 1 if classes else 0)
+        h = C.c_void_p()
+        L.check(L.load().tsem_serve_open(device, C.byref(o), C.byref(h)))
+        self.h = h
+        self.keep = {}  # request id -> the callbacks and arrays it uses
+
+    def submit(self, xyz, tet, M, problem, labels=None, dirichlet=0b000001, neumann=0b111110, topology=0,
+               qorder=0, tol=1e-10, mixed=1, precond=1):
+        import torch
+        dev = "cuda:%d" % self.device
+
+        def load(user, pts, n, out, stream):
+            try:
+                with torch.cuda.stream(torch.cuda.ExternalStream(stream, device=dev)):
+                    p = torch.as_tensor(_DeviceArray(pts, (n, 3)), device=dev)
+                    torch.as_tensor(_DeviceArray(out, (n,)), device=dev).copy_(problem.f(p))
+                return 0
+            except Exception:
+                return 1
+
+        def value(user, pts, n, out):
+            try:
+                p = torch.from_numpy(np.ctypeslib.as_array(pts, (n, 3)).copy()).to(dev)
+                np.ctypeslib.as_array(out, (n,))[:] = problem.u(p).cpu().numpy()
+                return 0
+            except Exception:
+                return 1
+
+        def flux(user, pts, nrm, n, out):
+            try:
+                p = torch.from_numpy(np.ctypeslib.as_array(pts, (n, 3)).copy()).to(dev)
+                q = torch.from_numpy(np.ctypeslib.as_array(nrm, (n, 3)).copy()).to(dev)
+                np.ctypeslib.as_array(out, (n,))[:] = problem.dudn(p, [q[:, d] for d in range(3)]).cpu().numpy()
+                return 0
+            except Exception:
+                return 1
+
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        tet = np.ascontiguousarray(tet, dtype=np.int32)
+        lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.int32)
+        cb = (_LOAD(load), _VALUE(value), _FLUX(flux))
+        r = ServeRequest(topology, M, qorder, len(xyz), len(tet), xyz.ctypes.data, tet.ctypes.data,
+                         None if lab is None else lab.ctypes.data, dirichlet, neumann, cb[0], cb[1], cb[2], None,
+                         tol, int(mixed), precond)
+        rid = C.c_int64()
+        L.check(L.load().tsem_serve_submit(self.h, C.byref(r), C.byref(rid)))
+        self.keep[rid.value] = cb
+        return rid.value
+
+    def poll(self, rid, wait=True):
+        """(done, x, result dict): x the solution (host) once done, and the
+        request released; (False, None, None) while it runs."""
+        res = ServeResult()
+        done = C.c_int()
+        rc = L.load().tsem_serve_poll(self.h, rid, 1 if wait else 0, None, C.byref(res), C.byref(done))
+        if not done.value:
+            L.check(rc)
+            return False, None, None
+        x = np.zeros(res.n_dof)
+        rc = L.load().tsem_serve_poll(self.h, rid, 1, x.ctypes.data, C.byref(res), C.byref(done))
+        self.keep.pop(rid, None)
+        L.check(rc)
+        return True, x, {k: getattr(res, k) for k, _ in ServeResult._fields_}
+
+    def run(self, requests):
+        """Serve Requests (n, p, seed, arrival; the Python Server's stream) on
+        the C ABI server: meshes are made first, each request is submitted at
+        its arrival time (relative to now) with a topology key per (n, p),
+        then all are collected.  Fills start/done/dofs/iterations/relres/
+        built/phases and returns (requests, summary) as Server.run does."""
+        from . import mesh, problems
+        data = {r.id: mesh.kuhn_cube(r.n, jitter=0.1, seed=r.seed) for r in requests}
+        order = sorted(requests, key=lambda r: (r.arrival, r.id))
+        ids = []
+        t0 = time.perf_counter()
+        for r in order:
+            dt = r.arrival - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            xyz, tet = data[r.id]
+            sub = time.perf_counter() - t0
+            ids.append((r, sub, self.submit(xyz, tet, r.p, problems.Manufactured(r.seed),
+                                            labels=mesh.cube_plane_marks(xyz), topology=1000 * r.n + r.p)))
+        for r, sub, rid in ids:
+            _, _, res = self.poll(rid)
+            r.start = sub + res["queued_s"]
+            r.done = sub + res["total_s"]
+            r.dofs, r.iterations, r.relres, r.built = res["n_dof"], res["iterations"], res["relres"], bool(res["built"])
+            r.phases = dict(space=res["space_s"], setup=res["setup_s"], solve=res["solve_s"])
+        wall = max(r.done for r in requests)
+        lat = np.array([r.done - r.arrival for r in requests])
+        return requests, dict(requests=len(requests), wall_s=wall, dofs_per_s=sum(r.dofs for r in requests) / wall,
+                              p50_ms=float(np.percentile(lat, 50)) * 1e3,
+                              p99_ms=float(np.percentile(lat, 99)) * 1e3,
+                              max_relres=max(r.relres for r in requests))
+
+    def close(self):
+        if self.h:
+            L.check(L.load().tsem_serve_close(self.h))
+            self.h = None
+            self.keep.clear()
