@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(256) k_gather_tiles(DevSpace S, int Kl, const 
   __shared__ int sm[TILE];
   const int tile = xcd_block(blockIdx.x, gridDim.x);
   const int e0 = tile * TILE;
-  // the face table rows of this space: w [R J], n [R], key [R J]
+  // the face table rows of this space: w [R J], n [R], key [R J], then the slot codes [K]
   extern __shared__ unsigned char gft_raw[];
   const int R = 6 * S.ndg2, J = S.ftc_J;
   T* cw = reinterpret_cast<T*>(gft_raw);
@@ -106,6 +106,8 @@ __global__ void __launch_bounds__(256) k_gather_tiles(DevSpace S, int Kl, const 
     ck[s] = S.ftc_key[s];
   }
   for (int s = threadIdx.x; s < R; s += 256) cn[s] = S.ftc_n[s];
+  int* sslot = ck + R * J;  // the slot codes of the local modes: no global load per entry
+  for (int s = threadIdx.x; s < S.K; s += 256) sslot[s] = S.slot[s];
   {
     const int32_t* src = reinterpret_cast<const int32_t*>(S.desc + e0);
     int32_t* dst = reinterpret_cast<int32_t*>(sd);
@@ -124,7 +126,7 @@ __global__ void __launch_bounds__(256) k_gather_tiles(DevSpace S, int Kl, const 
     if (e < S.nt && k < Kl) {
       const ElemDesc& D = sd[le];
       const int msk = sm[le];
-      int code = S.slot[k];
+      int code = sslot[k];
       int kind = code & 3, geo = (code >> 2) & 7, sub = code >> 5;
       if (kind == 0) {
         v = ((msk >> geo) & 1) ? T(0) : x[D.v[geo]];
@@ -316,7 +318,7 @@ static void run(const DevSpace& S, const OpCoef& op, const T* Dp, T* xt, const T
   const bool timed = op.Kl == 0;
   const int kind = sizeof(T) == 8 ? TK_GATHER64 : TK_GATHER32;
   int th = timed ? timing_begin(kind, st) : -1;
-  const size_t ftc = (size_t)6 * S.ndg2 * (S.ftc_J * (sizeof(T) + 4) + 4);
+  const size_t ftc = (size_t)6 * S.ndg2 * (S.ftc_J * (sizeof(T) + 4) + 4) + (size_t)S.K * 4;
   hipLaunchKernelGGL(k_gather_tiles<T>, dim3((unsigned)S.ntile), dim3(256), ftc, st, S, Kl, x, xt, op.constrained);
   timing_end(th, st);
   int nb = (S.nt + 16 * CT - 1) / (16 * CT);

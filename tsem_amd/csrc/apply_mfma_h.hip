@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(256) k_gather_tiles_h(DevSpace S, int Kl, cons
   __shared__ ElemDesc sd[TILE];
   __shared__ int sm[TILE];
   __shared__ float smax[TILE];
-  extern __shared__ float vf[];  // [K32][TILE], then the face table rows: w [R J], n [R], key [R J]
+  extern __shared__ float vf[];  // [K32][TILE], then the face table rows: w [R J], n [R], key [R J], slot codes [K]
   const int tile = xcd_block(blockIdx.x, gridDim.x);
   const int R = 6 * S.ndg2, J = S.ftc_J;
   float* cw = vf + (size_t)S.K32 * TILE;
@@ -72,6 +72,8 @@ __global__ void __launch_bounds__(256) k_gather_tiles_h(DevSpace S, int Kl, cons
   int* ck = cn + R;
   for (int s = threadIdx.x; s < R * J; s += 256) { cw[s] = S.ftc_w32[s]; ck[s] = S.ftc_key[s]; }
   for (int s = threadIdx.x; s < R; s += 256) cn[s] = S.ftc_n[s];
+  int* sslot = ck + R * J;  // the slot codes of the local modes: no global load per entry
+  for (int s = threadIdx.x; s < S.K; s += 256) sslot[s] = S.slot[s];
   const int e0 = tile * TILE;
   const int nvalid = min(TILE, S.nt - e0);
   {
@@ -90,7 +92,7 @@ __global__ void __launch_bounds__(256) k_gather_tiles_h(DevSpace S, int Kl, cons
     if (le < nvalid && k < Kl) {
       const ElemDesc& D = sd[le];
       const int msk = sm[le];
-      int code = S.slot[k];
+      int code = sslot[k];
       int kind = code & 3, geo = (code >> 2) & 7, sub = code >> 5;
       if (kind == 0) {
         v = ((msk >> geo) & 1) ? 0.0f : x[D.v[geo]];
@@ -529,7 +531,7 @@ void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, fl
   const int Kl = op.Kl > 0 ? op.Kl : S.K;
   const bool timed = op.Kl == 0;
   int th = timed ? timing_begin(TK_GATHER32, st) : -1;
-  const size_t ftc = (size_t)6 * S.ndg2 * (S.ftc_J * 8 + 4);
+  const size_t ftc = (size_t)6 * S.ndg2 * (S.ftc_J * 8 + 4) + (size_t)S.K * 4;
   hipLaunchKernelGGL(k_gather_tiles_h, dim3((unsigned)S.ntile), dim3(256), (size_t)S.K32 * TILE * 4 + ftc, st, S, Kl, x,
                      S.xhhi, S.xhlo, S.xhs, op.constrained);
   timing_end(th, st);

@@ -220,15 +220,18 @@ __global__ void __launch_bounds__(BLK) k_apply(DevSpace S, OpCoef op, const T* _
 
 // ---------------------------------------------------------------- scatter ----
 // block reduction of one double per thread (fixed tree)
+// block-wide sum in a fixed order: an xor butterfly inside each wave (every
+// lane ends with the same bits: each step adds the same two values, in either
+// order), then the wave sums in wave order; two barriers instead of one per
+// tree level
 __device__ inline double block_sum(double v, double* red) {
-  red[threadIdx.x] = v;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int s = BLK / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  double r = red[0];
-  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int w = 0; w < BLK / 64; ++w) r += red[w];
+  __syncthreads();  // red is reused by the next call
   return r;
 }
 
