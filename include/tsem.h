@@ -316,6 +316,16 @@ int tsem_l2_error_chunk(tsem_space* sp, const double* x_dev, int e0, int e1, con
  * the square root of the sum over all chunks. */
 int tsem_h1_error_chunk(tsem_space* sp, const double* x_dev, int e0, int e1, const double* g_dev, double* sum);
 
+/* Values of a solution at n arbitrary points: out[p] = u_h(pts[p]) (host
+ * arrays: pts n x 3, out n); elem[p] (host, may be NULL) is the element
+ * that holds the point, -1 (with out NaN) for a point outside the mesh.
+ * Points are located on the host (element boxes bucketed on a grid; of the
+ * elements holding a point on a shared face, edge or vertex, the lowest
+ * numbered one, where the field is continuous anyway), the basis is
+ * evaluated there and the sums run on the device.
+ * TSEM::calc_val_point(mesh, pos, u) (src/PointInfo.cpp:1143-1313). */
+int tsem_point_values(tsem_space* sp, const double* x_dev, int64_t n, const double* pts, double* out, int32_t* elem);
+
 /* LDA (unpolarized) at n points: Slater exchange + Perdew-Zunger 1981
  * correlation; vxc = d(rho eps_xc)/d rho, exc = rho eps_xc (device; either
  * output may be NULL).  The reference example uses VWN correlation

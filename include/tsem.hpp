@@ -392,6 +392,14 @@ class TSEM {
     return std::sqrt(tot);
   }
 
+  // calc_val_point(mesh, pos, u): u_h at the point pos[0..2] (NaN outside
+  // the mesh)
+  T calc_val_point(const RegularMesh&, const double* pos, const Vector<T>& u) {
+    double out = 0.0;
+    check(tsem_point_values(sp_, u.data(), 1, pos, &out, nullptr));
+    return (T)out;
+  }
+
   // write_coef(src, filename) / read_coef(dst, filename): 'M<TAB>n_dof', then
   // one value per line (14 significant digits); read_coef takes, per
   // geometry, the leading coefficients of a file of another order

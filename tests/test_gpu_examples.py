@@ -116,6 +116,11 @@ def test_poisson_mixbc_example_matches_python(ctx, M, tmp_path):
     # calc_l2_error_gradient through the C++ interface and through Python
     g_c, g_p = _field(out, "err_l2g"), _python_poisson.last["err_l2g"]
     assert abs(g_c - g_p) <= 1e-12 + 1e-7 * g_p, (g_c, g_p)
+    # calc_val_point: u_h at a point against the exact u, to the solution's error
+    uh_pt = float(re.search(r"u_h\(0.3,0.4,0.5\) = ([-+0-9.eE]+)", out).group(1))
+    u_pt = float(re.search(r" u\(0.3,0.4,0.5\) = ([-+0-9.eE]+)", out).group(1))
+    assert abs(u_pt - u(np.array([0.3, 0.4, 0.5]))) <= 1e-15
+    assert abs(uh_pt - u_pt) <= 50 * err_c, (uh_pt, u_pt, err_c)
     # write_coef: the C++ program's solution as text, read back here; the two
     # solves (host- and device-evaluated load) agree to the solver accuracy
     xyz, tet, _, _ = mesh.read_afepack_mesh(path)

@@ -99,6 +99,41 @@ def test_gradient_error_matches_oracle(pair):
     assert abs(got - ref) <= 1e-12 * ref, (got, ref)
 
 
+def test_point_values_match_oracle(pair):
+    """calc_val_point at random points: each point located in numpy by its
+    barycentric coordinates in every element (the lowest-numbered holder),
+    evaluated with the oracle's gathered coefficients and basis at the
+    reference point; a field linear in x, y, z (vertex coefficients only,
+    the barycentric vertex modes) is reproduced exactly; points outside the
+    mesh give NaN and element -1."""
+    import torch
+    osp, psp = pair
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(psp.ndof)
+    xd = torch.from_numpy(x).cuda()
+    pts = rng.uniform(0.0, 1.0, size=(300, 3))
+    got, elem = psp.point_values(xd, pts)
+    P = psp.xyz[psp.tet]
+    J = np.stack([P[:, k + 1] - P[:, 0] for k in range(3)], axis=-1)
+    Ji = np.linalg.inv(J)
+    xl = osp.gather(x)
+    for p, g, e in zip(pts, got, elem):
+        r = np.einsum("ekd,ed->ek", Ji, p[None, :] - P[:, 0])
+        lam = np.concatenate([1 - r.sum(1)[:, None], r], axis=1)
+        hold = np.nonzero(np.all(lam >= -1e-12, axis=1))[0]
+        assert e == hold[0], (p, e, hold)
+        phi, _ = osp.basis_at(r[e][None, :])
+        ref = xl[e] @ phi[0]
+        assert abs(g - ref) <= 1e-12 * max(1.0, abs(ref)), (p, g, ref)
+    a, c = np.array([0.7, -1.3, 2.1]), 0.4
+    xlin = np.zeros(psp.ndof)
+    xlin[psp.dof_start()[:psp.nv]] = psp.xyz @ a + c
+    lin, _ = psp.point_values(torch.from_numpy(xlin).cuda(), pts)
+    assert np.abs(lin - (pts @ a + c)).max() <= 1e-13
+    out, el = psp.point_values(xd, [[1.5, 0.5, 0.5], [-0.1, 0.2, 0.3], [0.5, 0.5, 1.0 + 1e-6]])
+    assert np.all(np.isnan(out)) and np.all(el == -1)
+
+
 def test_moments_of_unit_cube(pair):
     import torch
     _, psp = pair

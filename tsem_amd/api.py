@@ -127,6 +127,16 @@ class Space:
         L.check(L.load().tsem_space_maps(self.h, L.iptr(off), L.iptr(ind), L.dptr(val)))
         return off, ind, val
 
+    def point_values(self, x, pts):
+        """u_h at arbitrary points (host n x 3): (values, holding element or -1
+        and NaN outside the mesh); TSEM::calc_val_point."""
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(pts))
+        elem = np.zeros(len(pts), np.int32)
+        _ready()
+        L.check(L.load().tsem_point_values(self.h, _ptr(x), len(pts), L.dptr(pts), L.dptr(out), L.iptr(elem)))
+        return out, elem
+
     def slab_state(self):
         """(active, slab entries, largest chunk span, face row terms): active
         when the FP32 operator writes entity slabs (tsem_space_slab)."""

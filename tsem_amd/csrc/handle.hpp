@@ -86,6 +86,14 @@ struct tsem_space {
   DevBuf qphi, qphiT, qdphi, qw, vpot, qxl, qpart, qout, grad2;
   DevBuf Dh1hi, Dh1lo, Dh2hi, Dh2lo, xhhi, xhlo, xhs;  // fp16-split FP32 operator
   tsem::Slab slab;                                      // entity-slab layout (host tables)
+  // point location: element bounding boxes bucketed on a uniform grid
+  // (tsem_point_values; dropped when the coordinates change)
+  struct Locator {
+    bool valid = false;
+    int n[3] = {0, 0, 0};
+    double lo[3] = {0, 0, 0}, h[3] = {1, 1, 1};
+    std::vector<int32_t> ptr, elems;  // CSR over cells
+  } loc;
   DevBuf sl_meta, sl_chunk, sl_eoff, sl_perm, sl_tk, sl_tw, sl_tw32;
   // solver vectors
   DevBuf v64[6], v32[6];

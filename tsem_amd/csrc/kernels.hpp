@@ -237,6 +237,10 @@ void launch_l2_chunk(const DevSpace& S, int e0, int e1, const double* xl, const 
 // gradient error over elements [e0, e1), g: exact gradient at the points ((e1-e0) x nq x 3)
 void launch_h1_chunk(const DevSpace& S, int e0, int e1, const double* xl, const double* g, double* partial,
                      double* out, hipStream_t st);
+// out[p] = sum_i phi[p][i] xl[ep[p]][i] (NaN where ep[p] < 0): values at
+// located points, phi the basis there (n x K)
+void launch_point_values(int K, long n, const int32_t* ep, const double* phi, const double* xl, double* out,
+                         hipStream_t st);
 // LDA, Slater exchange + Perdew-Zunger correlation (unpolarized): potential
 // v_xc and energy density e_xc = rho (eps_x + eps_c) per point
 void launch_lda_pz(long n, const double* rho, double* vxc, double* exc, hipStream_t st);

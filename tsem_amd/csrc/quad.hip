@@ -309,4 +309,25 @@ void launch_diagonal(const DevSpace& S, const double* grad2, double* diag, doubl
   hipLaunchKernelGGL(k_diagonal, dim3((S.ndof + QB - 1) / QB), dim3(QB), 0, st, S, grad2, diag, inv, inv32);
 }
 
+// out[p] = sum_i phi[p][i] xl[e_p][i] in mode order; NaN where e_p < 0
+__global__ void __launch_bounds__(QB) k_point_values(int K, long n, const int32_t* __restrict__ ep,
+                                                     const double* __restrict__ phi, const double* __restrict__ xl,
+                                                     double* __restrict__ out) {
+  const long p = (long)blockIdx.x * QB + threadIdx.x;
+  if (p >= n) return;
+  const int e = ep[p];
+  if (e < 0) {
+    out[p] = __builtin_nan("");
+    return;
+  }
+  double s = 0.0;
+  for (int i = 0; i < K; ++i) s += phi[p * K + i] * xl[(size_t)e * K + i];
+  out[p] = s;
+}
+
+void launch_point_values(int K, long n, const int32_t* ep, const double* phi, const double* xl, double* out,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_point_values, dim3((unsigned)((n + QB - 1) / QB)), dim3(QB), 0, st, K, n, ep, phi, xl, out);
+}
+
 }  // namespace tsem

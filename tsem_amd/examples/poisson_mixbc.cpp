@@ -63,8 +63,11 @@ int main(int argc, char* argv[]) {
 
     const double err_l2 = tsem.calc_l2_error(solution, &u);
     const double err_l2g = tsem.calc_l2_error_gradient(mesh, solution, &grad_u);
-    std::printf("n_element = %d, M = %d, n_dof = %d, iterations = %d, err_l2 = %.16e, err_l2g = %.16e\n",
-                mesh.n_geometry(3), M, n_dof_total, its, err_l2, err_l2g);
+    double pos[3] = {0.3, 0.4, 0.5};
+    const double u_pt = tsem.calc_val_point(mesh, pos, solution);
+    std::printf("n_element = %d, M = %d, n_dof = %d, iterations = %d, err_l2 = %.16e, err_l2g = %.16e, "
+                "u_h(0.3,0.4,0.5) = %.16e, u(0.3,0.4,0.5) = %.16e\n",
+                mesh.n_geometry(3), M, n_dof_total, its, err_l2, err_l2g, u_pt, u(pos));
     if (argc > 5) tsem.write_coef(solution, argv[5]);  // optional: the solution's coefficients
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());

@@ -78,6 +78,7 @@ SIGNATURES = {
     "tsem_set_slab": [C.c_int],
     "tsem_space_slab_apply_host": [_vp, _vp, _vp],
     "tsem_space_slab": [_vp, _vp],
+    "tsem_point_values": [_vp, _vp, C.c_int64, _vp, _vp, _vp],
     "tsem_serve_open": [C.c_int, _vp, _vp],
     "tsem_serve_submit": [_vp, _vp, _vp],
     "tsem_serve_poll": [_vp, C.c_int64, C.c_int, _vp, _vp, _vp],
