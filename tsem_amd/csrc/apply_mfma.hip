@@ -108,6 +108,12 @@ __global__ void __launch_bounds__(256) k_gather_tiles(DevSpace S, int Kl, const 
   for (int s = threadIdx.x; s < R; s += 256) cn[s] = S.ftc_n[s];
   int* sslot = ck + R * J;  // the slot codes of the local modes: no global load per entry
   for (int s = threadIdx.x; s < S.K; s += 256) sslot[s] = S.slot[s];
+  // faces in a diagonal variant (bit v of sl_diag: each local face mode is
+  // +- the global face DOF of the same index, sign bit d of sl_sgn[v]) skip
+  // the table rows: one load, the same value as the one-term row gives
+  __shared__ uint32_t ssgn[6];
+  if (threadIdx.x < 6) ssgn[threadIdx.x] = S.sl_sgn[threadIdx.x];
+  const int dg = S.sl_diag;
   {
     const int32_t* src = reinterpret_cast<const int32_t*>(S.desc + e0);
     int32_t* dst = reinterpret_cast<int32_t*>(sd);
@@ -138,25 +144,30 @@ __global__ void __launch_bounds__(256) k_gather_tiles(DevSpace S, int Kl, const 
         v = ((msk >> 14) & 1) ? T(0) : x[D.t + sub];
       } else {
         int fo = D.fo[geo];
-        int row = (fo & 7) * S.ndg2 + sub;
         const bool fmask = (msk >> (10 + geo)) & 1;
-        T acc = T(0);
-        const int nterm = cn[row];
-        for (int j = 0; j < nterm; ++j) {
-          int key = ck[row * J + j];
-          int ie = key & 3, idx = key >> 2;
-          T w = cw[row * J + j];
-          T xv;
-          if (ie < 3) {
-            int le2 = (fo >> (3 + 3 * ie)) & 7;
-            if (((fo >> (12 + ie)) & 1) && (idx & 1)) w = -w;
-            xv = ((msk >> (4 + le2)) & 1) ? T(0) : x[(D.e[le2] & 0x7fffffff) + idx];
-          } else {
-            xv = fmask ? T(0) : x[D.f[geo] + idx];
+        if ((dg >> (fo & 7)) & 1) {
+          v = fmask ? T(0) : x[D.f[geo] + sub];
+          if (sub < 32 && ((ssgn[fo & 7] >> sub) & 1)) v = -v;
+        } else {
+          const int row = (fo & 7) * S.ndg2 + sub;
+          T acc = T(0);
+          const int nterm = cn[row];
+          for (int j = 0; j < nterm; ++j) {
+            int key = ck[row * J + j];
+            int ie = key & 3, idx = key >> 2;
+            T w = cw[row * J + j];
+            T xv;
+            if (ie < 3) {
+              int le2 = (fo >> (3 + 3 * ie)) & 7;
+              if (((fo >> (12 + ie)) & 1) && (idx & 1)) w = -w;
+              xv = ((msk >> (4 + le2)) & 1) ? T(0) : x[(D.e[le2] & 0x7fffffff) + idx];
+            } else {
+              xv = fmask ? T(0) : x[D.f[geo] + idx];
+            }
+            acc += xv * w;
           }
-          acc += xv * w;
+          v = acc;
         }
-        v = acc;
       }
     }
     out[r] = v;

@@ -107,7 +107,7 @@ struct DevSpace {
   // offsets and two orientation words, local mode -> slab row, transposed
   // face rows (slab.hpp)
   int sl_n1 = 0, sl_n2 = 0, sl_JT = 1, sl_nrow = 1, sl_chunk_max = 0;
-  int sl_diag = 1;                    // diagonal face variants (bit v)
+  int sl_diag = 0;                    // diagonal face variants (bit v; 0 without slab tables)
   uint32_t sl_sgn[6] = {0, 0, 0, 0, 0, 0};  // their minus signs per face DOF
   const int32_t* sl_meta = nullptr;
   const int32_t* sl_chunk = nullptr;

@@ -96,11 +96,17 @@ struct Entry {
   }
 };
 
-// pool key: caller topology, order, quadrature, sizes (a key of 0 is never shared)
-typedef std::tuple<int64_t, int, int, int, int> Key;
+// pool key: caller topology, order, quadrature, sizes (a key of 0 is never
+// shared: the request id takes the place of the element count)
+typedef std::tuple<int64_t, int, int, int, int64_t> Key;
 
+// a failed library call inside a request, with its status
+struct CallError : std::runtime_error {
+  int rc;
+  CallError(int r, const std::string& m) : std::runtime_error(m), rc(r) {}
+};
 void check(int rc, const char* what) {
-  if (rc != TSEM_OK) throw std::runtime_error(std::string(what) + ": " + tsem_last_error());
+  if (rc != TSEM_OK) throw CallError(rc, std::string(what) + ": " + tsem_last_error());
 }
 
 }  // namespace
@@ -120,7 +126,7 @@ struct tsem_server {
   double cost(const Job& j) const { return (double)j.req.nt * binom3(j.req.M); }
   bool small(const Job& j) const { return cost(j) <= o.small_cost; }
   Key key(const Job& j) const {
-    return Key(j.req.topology, j.req.M, j.req.qorder, j.req.nv, j.req.topology ? j.req.nt : -(int)j.id);
+    return Key(j.req.topology, j.req.M, j.req.qorder, j.req.nv, j.req.topology ? (int64_t)j.req.nt : -j.id);
   }
 
   // with the lock held: a busy pool entry for the job (an idle cached space,
@@ -340,6 +346,9 @@ void tsem_server::run(int w) {
     std::string err;
     try {
       solve(*j, *e);
+    } catch (const CallError& ex) {
+      status = ex.rc;
+      err = ex.what();
     } catch (const std::bad_alloc&) {
       status = TSEM_ENOMEM;
       err = "allocation failed";
