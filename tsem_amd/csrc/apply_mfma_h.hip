@@ -13,9 +13,9 @@
 //   xh[tile][k / 8][element][8] (the B-operand layout of the MFMA), scale
 //   xs[e] = max |x_e|.
 // k_elem_gemm_h: per block of NE = 16 CT elements,
-//   U = D X (GEMM 1), V = Ghat U per element (FP32, into LDS), exact per-
-//   element max |V| (LDS atomics), V / max split to fp16 hi/lo in LDS
-//   (element-major, the B-operand layout), Y = D^T V (GEMM 2), Y rescaled
+//   U = D X (GEMM 1), V = Ghat U per element (FP32, kept in registers),
+//   exact per-element max |V| (LDS atomics), V / max split to fp16 hi/lo in
+//   LDS (element-major, the B-operand layout), Y = D^T V (GEMM 2), Y rescaled
 //   and written element-major through an LDS transpose, or, from the same
 //   LDS tile, as entity-slab entries (the element's contribution to each of
 //   its global DOFs, written where the slab scatter reads it).
@@ -36,7 +36,6 @@ namespace {
 constexpr int WAVES = 4;
 constexpr int NT = 64 * WAVES;
 constexpr int RMAX = 12;  // row tiles of GEMM 2 (K <= 192)
-constexpr int GPT = 6;    // fp16 groups of V per thread in the split
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -53,8 +52,10 @@ __device__ inline void split8(const float* v, float inv, h8& hi, h8& lo) {
   }
 }
 
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 __device__ inline h8 ld8(const uint16_t* p) { return *reinterpret_cast<const h8*>(p); }
 __device__ inline void st8(uint16_t* p, h8 v) { *reinterpret_cast<h8*>(p) = v; }
+__device__ inline void st4(uint16_t* p, h4 v) { *reinterpret_cast<h4*>(p) = v; }
 }  // namespace
 
 // one block per 32-element tile
@@ -172,7 +173,6 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
   constexpr int NE = 16 * CT;
   constexpr int VS = NE + 1;  // odd row stride of the FP32 LDS tiles: no bank conflicts
   extern __shared__ unsigned char smem_raw[];
-  float* Vf = reinterpret_cast<float*>(smem_raw);                    // [3 Mp][VS]  (phase A)
   uint16_t* Vh = reinterpret_cast<uint16_t*>(smem_raw);              // hi [NE][R2], lo [NE][R2]  (phase C)
   unsigned int* colmax = reinterpret_cast<unsigned int*>(smem_raw + S.hlds_v);  // [NE] float bits
   const int K = S.K, K32 = S.K32, Mp = S.Mp, R2 = S.R2;
@@ -189,8 +189,14 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
   for (int s = tid; s < NE; s += NT) colmax[s] = 0u;
   __syncthreads();
 
-  // ---- phase A: GEMM 1 + mixing; wave w owns m tiles w, w + WAVES, ...
-  for (int t = wave; t < Mt; t += WAVES) {
+  // ---- phase A: GEMM 1 + mixing; wave w owns m tiles w + WAVES tp, tp <
+  // TPW; V stays in registers (vv) until the per-element maxima are complete
+  constexpr int TPW = 4 / CT;  // m tiles per wave: Mt <= 4 with CT = 4, <= 8 with CT = 2
+  float vv[TPW][CT][3][4];
+#pragma unroll
+  for (int tp = 0; tp < TPW; ++tp) {
+    const int t = wave + WAVES * tp;
+    if (t >= Mt) break;
     f4 acc[3][CT];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -241,48 +247,50 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
         float v0 = keep * (g0 * u0 + g3 * u1 + g4 * u2);
         float v1 = keep * (g3 * u0 + g1 * u1 + g5 * u2);
         float v2 = keep * (g4 * u0 + g5 * u1 + g2 * u2);
-        Vf[(size_t)(0 * Mp + m) * VS + col] = v0;
-        Vf[(size_t)(1 * Mp + m) * VS + col] = v1;
-        Vf[(size_t)(2 * Mp + m) * VS + col] = v2;
+        vv[tp][c][0][j] = v0;
+        vv[tp][c][1][j] = v1;
+        vv[tp][c][2][j] = v2;
         vmax = fmaxf(vmax, fmaxf(fabsf(v0), fmaxf(fabsf(v1), fabsf(v2))));
       }
       atomicMax(&colmax[col], __float_as_uint(vmax));
     }
   }
-  // rows of V beyond the m tiles computed stay zero for GEMM 2
-  for (int s = tid; s < 3 * Mp * NE; s += NT) {
-    int r = s / NE, m = r % Mp;
-    if (m >= 16 * Mt) Vf[(size_t)r * VS + s % NE] = 0.0f;
-  }
   __syncthreads();
 
-  // ---- phase B: V / max split to fp16, element-major with each derivative
-  // component padded to Mh (a multiple of 32) rows; in place: read all,
-  // then write
+  // ---- phase B: V / max split to fp16 hi/lo straight from the registers,
+  // element-major with each derivative component padded to Mh (a multiple
+  // of 32) rows; rows from 16 Mt on are zero for GEMM 2
   const int Mh = S.Mh;
-  const int ngroups = 3 * (Mh / 8) * NE;
-  float vbuf[GPT][8];
 #pragma unroll
-  for (int q = 0; q < GPT; ++q) {
-    int g = tid + q * NT;
-    if (g < ngroups) {
-      int e = g % NE, r8 = g / NE;
-      int a = r8 / (Mh / 8), m0 = 8 * (r8 % (Mh / 8));
+  for (int tp = 0; tp < TPW; ++tp) {
+    const int t = wave + WAVES * tp;
+    if (t >= Mt) break;
 #pragma unroll
-      for (int t = 0; t < 8; ++t) vbuf[q][t] = (m0 + t < Mp) ? Vf[(size_t)(a * Mp + m0 + t) * VS + e] : 0.0f;
+    for (int c = 0; c < CT; ++c) {
+      const int col = 16 * c + lr;
+      const float m = __uint_as_float(colmax[col]);
+      const float inv = m > 0.0f ? 1.0f / m : 0.0f;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        h4 hi, lo;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float sv = vv[tp][c][a][j] * inv;
+          const _Float16 h = (_Float16)sv;
+          hi[j] = h;
+          lo[j] = (_Float16)(sv - (float)h);
+        }
+        const size_t o = (size_t)a * Mh + 16 * t + 4 * lk;
+        st4(Vh + (size_t)col * R2 + o, hi);
+        st4(Vh + (size_t)(NE + col) * R2 + o, lo);
+      }
     }
   }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < GPT; ++q) {
-    int g = tid + q * NT;
-    if (g < ngroups) {
-      int e = g % NE, r8 = g / NE;
-      float m = __uint_as_float(colmax[e]);
-      h8 hi, lo;
-      split8(vbuf[q], m > 0.0f ? 1.0f / m : 0.0f, hi, lo);
-      st8(Vh + (size_t)e * R2 + 8 * r8, hi);
-      st8(Vh + (size_t)(NE + e) * R2 + 8 * r8, lo);
+  {
+    const int zr = (Mh - 16 * Mt) / 4;  // groups of 4 zero rows per component
+    for (int s = tid; s < 2 * NE * 3 * zr; s += NT) {
+      const int g = s % zr, r = s / zr, a = r % 3, row = r / 3;  // row: hi rows 0..NE-1, lo rows NE..2NE-1
+      st4(Vh + (size_t)row * R2 + (size_t)a * Mh + 16 * Mt + 4 * g, h4{0, 0, 0, 0});
     }
   }
   __syncthreads();
@@ -532,9 +540,8 @@ bool slab_h_fits(const DevSpace& S) {
 size_t gemm_h_lds(const DevSpace& S) { return gemm_h_colmax_offset(S) + (size_t)16 * gemm_h_ct(S) * 4; }
 
 bool mfma_h_supported(const DevSpace& S) {
-  const int NE = 16 * gemm_h_ct(S);
   return S.Dh1hi != nullptr && S.xhhi != nullptr && S.K <= 16 * RMAX && S.Mp <= 128 && S.K32 <= 192 &&
-         3 * (S.Mh / 8) * NE <= GPT * NT && gemm_h_lds(S) <= 160 * 1024;
+         S.Mp / 16 <= WAVES * (4 / gemm_h_ct(S)) && gemm_h_lds(S) <= 160 * 1024;
 }
 
 void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, float* yl, bool slab, hipStream_t st) {
