@@ -187,6 +187,19 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
   const int KB = K32 / 8;
   const float cst = (float)op.stiff * S.dinv;  // undo the factor scaling of GEMM 1
   for (int s = tid; s < NE; s += NT) colmax[s] = 0u;
+  // slab form: the block's entity offsets and orientation words are loaded
+  // now, into registers, and staged into LDS only after GEMM 2, so that
+  // their latency passes behind the GEMMs
+  constexpr int EOQ = (NE * 17 + NT - 1) / NT;
+  int32_t eopre[EOQ];
+  {
+    const int nv0 = min(NE, S.nt - e0);
+#pragma unroll
+    for (int q = 0; q < EOQ; ++q) {
+      const int s2 = tid + q * NT;
+      eopre[q] = (slab && s2 < nv0 * 17) ? S.sl_eoff[(size_t)e0 * 17 + s2] : 0;
+    }
+  }
   __syncthreads();
 
   // ---- phase A: GEMM 1 + mixing; wave w owns m tiles w + WAVES tp, tp <
@@ -364,7 +377,9 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
     stw = reinterpret_cast<float*>(seo + NE * 17);
     stk = reinterpret_cast<uint16_t*>(stw + nrj);
     stn = stk + nrj;
-    for (int s = tid; s < NE * 17; s += NT) seo[s] = (s < nval * 17) ? S.sl_eoff[(size_t)e0 * 17 + s] : 0;
+#pragma unroll
+    for (int q = 0; q < EOQ; ++q)
+      if (tid + q * NT < NE * 17) seo[tid + q * NT] = eopre[q];
     for (int s = tid; s < nrj; s += NT) { stw[s] = S.sl_tw32[s]; stk[s] = (uint16_t)((S.sl_tk[s] & 255) * VS); }
     for (int s = tid; s < nrow; s += NT) stn[s] = (uint16_t)(S.sl_tk[s * JT] >> 8);
     int32_t* work = reinterpret_cast<int32_t*>(smem_raw + slab_tab_offset(K, NE) + slab_tab_bytes(S, NE));
