@@ -600,12 +600,17 @@ __global__ void __launch_bounds__(BLK) k_final_alpha(const double* __restrict__ 
 __global__ void __launch_bounds__(BLK) k_final_beta(const double* __restrict__ partial, int nb, int with_rz,
                                                     CgScalars* sc) {
   __shared__ double red[BLK];
-  double v[3];
-  for (int q = 0; q < 3; ++q) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += BLK) s += partial[(size_t)q * nb + b];
-    v[q] = block_sum(s, red);
+  // the three sums in one pass over the partials (each in block order)
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int b = threadIdx.x; b < nb; b += BLK) {
+    s0 += partial[b];
+    s1 += partial[(size_t)nb + b];
+    s2 += partial[2 * (size_t)nb + b];
   }
+  double v[3];
+  v[0] = block_sum(s0, red);
+  v[1] = block_sum(s1, red);
+  v[2] = block_sum(s2, red);
   if (threadIdx.x == 0) {
     double rz = with_rz ? v[0] : sc->tmp[0];
     sc->rz_new = rz;
