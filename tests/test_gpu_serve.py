@@ -62,7 +62,7 @@ def test_foreground_and_background_classes(ctx):
     alone = serve.Server(slots=1, keep_solutions=True, classes=False)
     for r in serve.make_requests(8, seed=5, n_range=(3, 6), p_range=(3, 6)):
         (one,), _ = alone.run([r])
-        assert np.array_equal(one.solution, got[r.id].solution), r
+        assert np.array_equal(one.solution, got[r.id].solution), (r, one.relres, got[r.id].relres)
     alone.close()
 
 

@@ -55,7 +55,10 @@ def _ptr(t):
 
 
 def _ready():
-    """Finish pending torch work before the library (own stream) reads it."""
+    """Finish pending torch work before the library (own stream) reads a
+    tensor, or writes one that torch's allocator has just handed out: a block
+    freed by torch may still be written by work queued on torch's stream (by
+    any thread), which is not ordered before the library's stream."""
     import torch
     torch.cuda.current_stream().synchronize()
 
@@ -253,6 +256,7 @@ class Space:
                 pause()
             e1 = min(self.nt, e0 + chunk)
             pts = torch.empty((e1 - e0, self.nq3, 3), dtype=torch.float64, device=self.device)
+            _ready()
             L.check(lib.tsem_qp_coords(self.h, e0, e1, _ptr(pts)))
             fq = f(pts).contiguous()
             _ready()
@@ -279,6 +283,7 @@ class Space:
         import torch
         e1 = self.nt if e1 is None else e1
         pts = torch.empty((e1 - e0, self.nq3, 3), dtype=torch.float64, device=self.device)
+        _ready()
         L.check(L.load().tsem_qp_coords(self.h, e0, e1, _ptr(pts)))
         return pts
 

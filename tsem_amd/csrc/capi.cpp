@@ -34,6 +34,14 @@ static bool g_pcg_graphs = [] {
   return !(v && v[0] == '0');
 }();
 
+// alpha formed inside the CG update kernel rather than by its own
+// single-block reduction (TSEM_FUSED_ALPHA=0 keeps the separate kernel; the
+// two give the same alpha bitwise)
+static const bool g_fused_alpha = [] {
+  const char* v = getenv("TSEM_FUSED_ALPHA");
+  return !(v && v[0] == '0');
+}();
+
 const char* tsem_last_error(void) { return g_err.c_str(); }
 
 // Cooperative solve classes (process wide): while any foreground solve runs,
@@ -1031,16 +1039,18 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
   HIPCHK(hipMemsetAsync(p, 0, (size_t)n * sizeof(T), st));
   HIPCHK(hipMemsetAsync(q, 0, (size_t)n * sizeof(T), st));
   // x += alpha p, r += alpha q, z = P r; partials [r.z | |r|_1 | |r|_2^2] then
-  // beta = r.z / r.z_old in one final reduction
-  auto update_and_precondition = [&]() {
-    launch_cg_update<T>(d, n, x, r, precond == 4 ? nullptr : z, p, q, sc, part, precond == 4 ? 0 : 1, st);
+  // beta = r.z / r.z_old in one final reduction.  With pq_part, alpha is
+  // formed from the p.q partials inside the update; else it is read from sc
+  auto update_and_precondition = [&](const double* pq_part) {
+    launch_cg_update<T>(d, n, x, r, precond == 4 ? nullptr : z, p, q, sc, part, precond == 4 ? 0 : 1, pq_part,
+                        st);
     if (precond == 4) {
       sp->pmg->vcycle<T>(r, z, smooth, st);
       launch_dot<T>(n, r, z, part, st);  // replaces the r.z partials
     }
     launch_final_beta(part, nb, 1, sc, st);
   };
-  update_and_precondition();
+  update_and_precondition(nullptr);
   // p = -z
   launch_axpby<T>(n, T(-1), z, T(0), p, st);
   CgScalars hs{};
@@ -1053,10 +1063,18 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
   int it = 0;
   static const bool trace = getenv("TSEM_PCG_TRACE") != nullptr;
   if (trace) fprintf(stderr, "[tsem pcg] start res %.3e bnorm %.3e rz %.3e\n", res, bnorm, hs.rz);
+  // the p.q partials go to the fourth quarter of the partial buffer, which
+  // the update's own partials do not reach
+  double* part_pq = part + 3 * (size_t)nb;
   auto iteration = [&]() {
-    launch_apply<T>(d, op, p, q, yl, part, st);
-    launch_final_alpha(part, nb, sc, st);
-    update_and_precondition();
+    if (g_fused_alpha) {
+      launch_apply<T>(d, op, p, q, yl, part_pq, st);
+      update_and_precondition(part_pq);
+    } else {
+      launch_apply<T>(d, op, p, q, yl, part, st);
+      launch_final_alpha(part, nb, sc, st);
+      update_and_precondition(nullptr);
+    }
     launch_p_update<T>(n, p, z, sc, st);
   };
   auto& G = (sizeof(T) == 8) ? sp->graph64 : sp->graph32;
@@ -1118,7 +1136,7 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
       auto refresh = [&](bool restart) {
         static const double zero = 0.0;
         HIPCHK(hipMemcpyAsync(&sc->alpha, &zero, sizeof(double), hipMemcpyHostToDevice, st));
-        update_and_precondition();
+        update_and_precondition(nullptr);
         if (restart) launch_axpby<T>(n, T(-1), z, T(0), p, st);
         HIPCHK(hipMemcpyAsync(&hs, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));

@@ -357,8 +357,9 @@ __global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __r
 // loads; DOF g then adds its m entries, spaced by its geometry's DOF count,
 // in element order.  The slab already carries the signs and face weights,
 // so there are no codes, weights or indirect loads.  A block takes SP
-// consecutive chunks at a time and issues all their loads before waiting
-// (the loads in flight, not the instructions, bound this kernel).
+// consecutive chunks at a time and issues all their loads (the spans, and
+// each DOF's input value and elimination flag) before waiting (the loads in
+// flight, not the instructions, bound this kernel).
 // Eliminated rows, partial sums and the XCD chunk ranges are those of
 // k_scatter_chunks.
 constexpr int SP = 4;   // chunks per step
@@ -378,14 +379,20 @@ __global__ void __launch_bounds__(BLK) k_scatter_slab(DevSpace S, const T* __res
   const int nbx = gridDim.x / NXCD + (xcd < (int)(gridDim.x % NXCD) ? 1 : 0);
   const int q0 = (int)((long long)xcd * ngr / nx), q1 = (int)((long long)(xcd + 1) * ngr / nx);
   const int n1 = S.sl_n1, n2 = S.sl_n2, cm = S.sl_chunk_max;
+  const bool need_x = partial || constrained;
   for (int q = q0 + jb; q < q1; q += nbx) {
     int meta[SP];
     T v[SP][SU];
+    T xg[SP];       // the input value at the DOF: partial sum, eliminated row
+    uint8_t cg[SP];  // 1 where the DOF is eliminated
 #pragma unroll
     for (int p = 0; p < SP; ++p) {
       const int c = q * SP + p;
       const int g = c * SCH + (int)threadIdx.x;
-      meta[p] = (c < nch && g < S.ndof) ? S.sl_meta[g] : 0;
+      const bool own = c < nch && g < S.ndof;
+      meta[p] = own ? S.sl_meta[g] : 0;
+      xg[p] = (own && need_x) ? xin[g] : T(0);
+      cg[p] = (own && constrained) ? S.cmask[g] : (uint8_t)0;
       const int lo = c < nch ? S.sl_chunk[2 * c] : 0, len = c < nch ? S.sl_chunk[2 * c + 1] : 0;
 #pragma unroll
       for (int u = 0; u < SU; ++u) {
@@ -410,9 +417,9 @@ __global__ void __launch_bounds__(BLK) k_scatter_slab(DevSpace S, const T* __res
         const T* row = vals + p * cm + rel;
         T acc = T(0);
         for (int k = 0; k < m; ++k) acc += row[k * stride];
-        if (constrained && S.cmask[g]) acc = (T)S.diag[g] * xin[g];
+        if (cg[p]) acc = (T)S.diag[g] * xg[p];
         y[g] = acc;
-        if (partial) d += (double)xin[g] * (double)acc;
+        if (partial) d += (double)xg[p] * (double)acc;
       }
     }
     __syncthreads();  // vals is refilled by the next step
@@ -640,15 +647,33 @@ void launch_scalar_alpha(CgScalars* sc, hipStream_t st) { hipLaunchKernelGGL(k_a
 void launch_scalar_beta(CgScalars* sc, hipStream_t st) { hipLaunchKernelGGL(k_beta, dim3(1), dim3(1), 0, st, sc); }
 
 // x += alpha p ; r += alpha q ; z = r / diag (Jacobi) or z = r ;
-// partial sums of r.z, |r|_1, |r|_2^2  (src/TSEMSolver.cpp:1846-1897)
+// partial sums of r.z, |r|_1, |r|_2^2  (src/TSEMSolver.cpp:1846-1897).
+// With pq_part, every block first sums the nb partials of p.q itself, in the
+// order and with the code of k_final_alpha (so every block gets the same,
+// bitwise, alpha = rz / pq), and block 0 records pq and alpha; otherwise
+// alpha is read from sc.  pq_part must not overlap partial.
 template <typename T>
 __global__ void __launch_bounds__(BLK) k_cg_update(DevSpace S, int n, T* __restrict__ x, T* __restrict__ r,
                                                    T* __restrict__ z, const T* __restrict__ p,
-                                                   const T* __restrict__ q, const CgScalars* __restrict__ sc,
-                                                   double* __restrict__ partial, int jac) {
+                                                   const T* __restrict__ q, CgScalars* __restrict__ sc,
+                                                   double* __restrict__ partial, int jac,
+                                                   const double* __restrict__ pq_part, int pq_nb) {
   __shared__ double red[BLK];
   double a = 0, b = 0, c = 0;
-  const T alpha = (T)sc->alpha;
+  double alpha_d;
+  if (pq_part) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < pq_nb; k += BLK) s += pq_part[k];
+    const double pq = block_sum(s, red);
+    alpha_d = sc->rz / pq;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->pq = pq;
+      sc->alpha = alpha_d;
+    }
+  } else {
+    alpha_d = sc->alpha;
+  }
+  const T alpha = (T)alpha_d;
   for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
     x[i] += alpha * p[i];
     T ri = r[i] + alpha * q[i];
@@ -677,14 +702,15 @@ __global__ void __launch_bounds__(BLK) k_cg_update(DevSpace S, int n, T* __restr
 
 template <typename T>
 void launch_cg_update(const DevSpace& S, int n, T* x, T* r, T* z, const T* p, const T* q, CgScalars* sc,
-                      double* partial, int jac, hipStream_t st) {
+                      double* partial, int jac, const double* pq_part, hipStream_t st) {
   int nb = reduce_blocks(n);
-  hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(BLK), 0, st, S, n, x, r, z, p, q, sc, partial, jac);
+  hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(BLK), 0, st, S, n, x, r, z, p, q, sc, partial, jac, pq_part,
+                     nb);
 }
 template void launch_cg_update<double>(const DevSpace&, int, double*, double*, double*, const double*, const double*,
-                                       CgScalars*, double*, int, hipStream_t);
+                                       CgScalars*, double*, int, const double*, hipStream_t);
 template void launch_cg_update<float>(const DevSpace&, int, float*, float*, float*, const float*, const float*,
-                                      CgScalars*, double*, int, hipStream_t);
+                                      CgScalars*, double*, int, const double*, hipStream_t);
 
 // p = beta p - z
 template <typename T>

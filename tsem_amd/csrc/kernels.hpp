@@ -177,9 +177,12 @@ void launch_scatter(const DevSpace& S, const T* yl, T* y, hipStream_t st);
 int reduce_blocks(int n);
 void launch_reduce_final(const double* partial, int nb, int nvals, double* out, hipStream_t st);
 
+// pq_part: the reduce_blocks(n) partials of p.q; alpha = rz / pq is then
+// formed inside the update (no separate final reduction); nullptr uses
+// sc->alpha
 template <typename T>
 void launch_cg_update(const DevSpace& S, int n, T* x, T* r, T* z, const T* p, const T* q, CgScalars* sc,
-                      double* partial, int precond_jacobi, hipStream_t st);
+                      double* partial, int precond_jacobi, const double* pq_part, hipStream_t st);
 template <typename T>
 void launch_p_update(int n, T* p, const T* z, CgScalars* sc, hipStream_t st);
 void launch_scalar_alpha(CgScalars* sc, hipStream_t st);
