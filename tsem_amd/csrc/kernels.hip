@@ -314,9 +314,16 @@ __global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __r
   const int xcd = blockIdx.x % NXCD, jb = blockIdx.x / NXCD;
   const int nbx = gridDim.x / NXCD + (xcd < (int)(gridDim.x % NXCD) ? 1 : 0);
   const int c0 = (int)((long long)xcd * nch / nx), c1 = (int)((long long)(xcd + 1) * nch / nx);
+  const bool need_x = partial || constrained;
   for (int c = c0 + jb; c < c1; c += nbx) {
     const int g0 = c * SCH, g1 = min(S.ndof, g0 + SCH);
     const int t0 = S.sc_ptr[g0], t1 = S.sc_ptr[g1];
+    // the row's bounds, input value and elimination flag are loaded with the
+    // terms, not after them
+    const int g = g0 + (int)threadIdx.x;
+    const int ra = g < g1 ? S.sc_ptr[g] : 0, rb = g < g1 ? S.sc_ptr[g + 1] : 0;
+    const T xg = (g < g1 && need_x) ? xin[g] : T(0);
+    const uint8_t cg = (g < g1 && constrained) ? S.cmask[g] : (uint8_t)0;
     for (int t = t0 + threadIdx.x; t < t1; t += U * BLK) {
       int cc[U];
       T v[U];
@@ -336,14 +343,13 @@ __global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __r
       }
     }
     __syncthreads();
-    const int g = g0 + (int)threadIdx.x;
     if (g < g1) {
-      const int a = S.sc_ptr[g] - t0, b = S.sc_ptr[g + 1] - t0;
+      const int a = ra - t0, b = rb - t0;
       T acc = T(0);
       for (int t = a; t < b; ++t) acc += vals[t];
-      if (constrained && S.cmask[g]) acc = (T)S.diag[g] * xin[g];
+      if (cg) acc = (T)S.diag[g] * xg;
       y[g] = acc;
-      if (partial) d += (double)xin[g] * (double)acc;
+      if (partial) d += (double)xg * (double)acc;
     }
     __syncthreads();  // vals is refilled by the next chunk
   }
