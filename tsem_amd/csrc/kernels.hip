@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __r
     const int g = g0 + (int)threadIdx.x;
     const int ra = g < g1 ? S.sc_ptr[g] : 0, rb = g < g1 ? S.sc_ptr[g + 1] : 0;
     const T xg = (g < g1 && need_x) ? xin[g] : T(0);
-    const uint8_t cg = (g < g1 && constrained) ? S.cmask[g] : (uint8_t)0;
+    const uint8_t cg = S.cmask[g < g1 ? g : 0];  // unconditional, as in k_scatter_slab
     for (int t = t0 + threadIdx.x; t < t1; t += U * BLK) {
       int cc[U];
       T v[U];
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __r
       const int a = ra - t0, b = rb - t0;
       T acc = T(0);
       for (int t = a; t < b; ++t) acc += vals[t];
-      if (cg) acc = (T)S.diag[g] * xg;
+      if (constrained && cg) acc = (T)S.diag[g] * xg;
       y[g] = acc;
       if (partial) d += (double)xg * (double)acc;
     }
@@ -369,6 +369,7 @@ __global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __r
 // Eliminated rows, partial sums and the XCD chunk ranges are those of
 // k_scatter_chunks.
 constexpr int SP = 4;   // chunks per step
+static_assert(SP % 2 == 0 && SLAB_CHUNK_GROUP % SP == 0, "a step reads whole int4 pairs of its group");
 constexpr int SU = 4;   // span entries per thread and chunk: spans <= SU * BLK
 template <typename T>
 __global__ void __launch_bounds__(BLK) k_scatter_slab(DevSpace S, const T* __restrict__ sl, T* __restrict__ y,
@@ -386,6 +387,20 @@ __global__ void __launch_bounds__(BLK) k_scatter_slab(DevSpace S, const T* __res
   const int q0 = (int)((long long)xcd * ngr / nx), q1 = (int)((long long)(xcd + 1) * ngr / nx);
   const int n1 = S.sl_n1, n2 = S.sl_n2, cm = S.sl_chunk_max;
   const bool need_x = partial || constrained;
+  // span bounds of the step about to run; the next step's are loaded while
+  // this one's values are in flight.  The table is padded with empty chunks
+  // to whole groups, so a step's bounds are read without a test
+  int lo[SP], len[SP];
+  auto bounds = [&](int q) {
+    const int4* cb = reinterpret_cast<const int4*>(S.sl_chunk) + (size_t)q * (SP / 2);
+#pragma unroll
+    for (int h = 0; h < SP / 2; ++h) {
+      const int4 w = cb[h];
+      lo[2 * h] = w.x; len[2 * h] = w.y;
+      lo[2 * h + 1] = w.z; len[2 * h + 1] = w.w;
+    }
+  };
+  if (q0 + jb < q1) bounds(q0 + jb);
   for (int q = q0 + jb; q < q1; q += nbx) {
     int meta[SP];
     T v[SP][SU];
@@ -398,14 +413,17 @@ __global__ void __launch_bounds__(BLK) k_scatter_slab(DevSpace S, const T* __res
       const bool own = c < nch && g < S.ndof;
       meta[p] = own ? S.sl_meta[g] : 0;
       xg[p] = (own && need_x) ? xin[g] : T(0);
-      cg[p] = (own && constrained) ? S.cmask[g] : (uint8_t)0;
-      const int lo = c < nch ? S.sl_chunk[2 * c] : 0, len = c < nch ? S.sl_chunk[2 * c + 1] : 0;
+      // read unconditionally (DOF 0 stands in past the end): a flag loaded
+      // under a condition is turned into a lane mask at once, and the wait
+      // for it would hold back the loads after it
+      cg[p] = S.cmask[own ? g : 0];
 #pragma unroll
       for (int u = 0; u < SU; ++u) {
         const int t = (int)threadIdx.x + u * BLK;
-        v[p][u] = (t < len) ? sl[lo + t] : T(0);
+        v[p][u] = (t < len[p]) ? sl[lo[p] + t] : T(0);
       }
     }
+    if (q + nbx < q1) bounds(q + nbx);
 #pragma unroll
     for (int p = 0; p < SP; ++p)
 #pragma unroll
@@ -423,7 +441,7 @@ __global__ void __launch_bounds__(BLK) k_scatter_slab(DevSpace S, const T* __res
         const T* row = vals + p * cm + rel;
         T acc = T(0);
         for (int k = 0; k < m; ++k) acc += row[k * stride];
-        if (cg[p]) acc = (T)S.diag[g] * xg[p];
+        if (constrained && cg[p]) acc = (T)S.diag[g] * xg[p];
         y[g] = acc;
         if (partial) d += (double)xg[p] * (double)acc;
       }

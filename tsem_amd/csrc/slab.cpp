@@ -129,7 +129,7 @@ void build_slab(const Space& S, Slab& out) {
 
   // per DOF: offset in its chunk's span, count, dimension
   const int64_t nch = (ndof + SLAB_ROWS - 1) / SLAB_ROWS;
-  out.chunk.assign(2 * nch, 0);
+  out.chunk.assign(2 * ((nch + SLAB_CHUNK_GROUP - 1) / SLAB_CHUNK_GROUP * SLAB_CHUNK_GROUP), 0);
   out.meta.assign(ndof, 0);
   std::vector<int64_t> first(ndof);
   for (int64_t g = 0; g < ndof; ++g) first[g] = -1;

@@ -22,6 +22,9 @@
 namespace tsem {
 
 constexpr int SLAB_ROWS = 256;  // DOFs per scatter chunk
+// the chunk table is padded with empty chunks to a multiple of this many, so
+// that a scatter step reads the bounds of a whole group without a test
+constexpr int SLAB_CHUNK_GROUP = 8;
 constexpr int SLAB_JMAX = 8;    // most terms of a transposed face row
 
 // rows of the transposed face tables: face DOF rows (v, i), then edge DOF
@@ -35,7 +38,8 @@ struct Slab {
   int n1 = 0, n2 = 0, n3 = 0, JT = 1, nrow = 1;
   int chunk_max = 0;            // largest slab span of a chunk
   std::vector<int32_t> meta;    // per DOF: offset in its chunk's span | m << 20 | dim << 30
-  std::vector<int32_t> chunk;   // per chunk of SLAB_ROWS DOFs: span start, span length
+  std::vector<int32_t> chunk;   // per chunk of SLAB_ROWS DOFs: span start, span length (then
+                                // empty chunks up to a multiple of SLAB_CHUNK_GROUP)
   // per element (17 ints): 15 entity offsets (v0..3, e0..5, f0..3,
   // interior), orientation words ow and ow2 (slab_edge)
   std::vector<int32_t> eoff;
