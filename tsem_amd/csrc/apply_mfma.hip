@@ -101,26 +101,61 @@ __global__ void __launch_bounds__(256) k_gather_tiles(DevSpace S, int Kl, const 
   T* cw = reinterpret_cast<T*>(gft_raw);
   int* cn = reinterpret_cast<int*>(cw + (size_t)R * J);
   int* ck = cn + R;
-  for (int s = threadIdx.x; s < R * J; s += 256) {
-    cw[s] = (sizeof(T) == 8) ? (T)S.ftc_w[s] : (T)S.ftc_w32[s];
-    ck[s] = S.ftc_key[s];
-  }
-  for (int s = threadIdx.x; s < R; s += 256) cn[s] = S.ftc_n[s];
   int* sslot = ck + R * J;  // the slot codes of the local modes: no global load per entry
-  for (int s = threadIdx.x; s < S.K; s += 256) sslot[s] = S.slot[s];
   // faces in a diagonal variant (bit v of sl_diag: each local face mode is
   // +- the global face DOF of the same index, sign bit d of sl_sgn[v]) skip
   // the table rows: one load, the same value as the one-term row gives
   __shared__ uint32_t ssgn[6];
-  if (threadIdx.x < 6) ssgn[threadIdx.x] = S.sl_sgn[threadIdx.x];
   const int dg = S.sl_diag;
+  // the set-up loads are all issued before any of them is waited for (as in
+  // k_gather_tiles_h); table entries past the first TQ per thread follow in
+  // plain loops
   {
+    const int tid = threadIdx.x;
+    const int nvalid = min(TILE, S.nt - e0);
+    constexpr int W = sizeof(ElemDesc) / 4;
+    constexpr int DQ = (TILE * W + 255) / 256;
+    constexpr int TQ = 2;
     const int32_t* src = reinterpret_cast<const int32_t*>(S.desc + e0);
     int32_t* dst = reinterpret_cast<int32_t*>(sd);
-    constexpr int W = sizeof(ElemDesc) / 4;
-    int nvalid = min(TILE, S.nt - e0);
-    for (int s = threadIdx.x; s < TILE * W; s += 256) dst[s] = (s / W < nvalid) ? src[s] : 0;
-    for (int s = threadIdx.x; s < TILE; s += 256) sm[s] = (constrained && s < nvalid) ? S.gmask[e0 + s] : 0;
+    int32_t dv[DQ];
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+      const int s = tid + 256 * q;
+      dv[q] = (s < TILE * W && s / W < nvalid) ? src[s] : 0;
+    }
+    const int gmv = (constrained && tid < nvalid) ? S.gmask[e0 + tid] : 0;
+    T cwv[TQ];
+    int ckv[TQ];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) {
+      const int s = tid + 256 * q;
+      cwv[q] = s < R * J ? ((sizeof(T) == 8) ? (T)S.ftc_w[s] : (T)S.ftc_w32[s]) : T(0);
+      ckv[q] = s < R * J ? S.ftc_key[s] : 0;
+    }
+    const int cnv = tid < R ? S.ftc_n[tid] : 0;
+    const int slv = tid < S.K ? S.slot[tid] : 0;
+    const uint32_t sgv = tid < 6 ? S.sl_sgn[tid] : 0u;
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+      const int s = tid + 256 * q;
+      if (s < TILE * W) dst[s] = dv[q];
+    }
+    if (tid < TILE) sm[tid] = gmv;
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) {
+      const int s = tid + 256 * q;
+      if (s < R * J) { cw[s] = cwv[q]; ck[s] = ckv[q]; }
+    }
+    for (int s = tid + 256 * TQ; s < R * J; s += 256) {
+      cw[s] = (sizeof(T) == 8) ? (T)S.ftc_w[s] : (T)S.ftc_w32[s];
+      ck[s] = S.ftc_key[s];
+    }
+    if (tid < R) cn[tid] = cnv;
+    for (int s = tid + 256; s < R; s += 256) cn[s] = S.ftc_n[s];
+    if (tid < S.K) sslot[tid] = slv;
+    for (int s = tid + 256; s < S.K; s += 256) sslot[s] = S.slot[s];
+    if (tid < 6) ssgn[tid] = sgv;
   }
   __syncthreads();
   const int Kp = S.Kp;

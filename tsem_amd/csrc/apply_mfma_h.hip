@@ -215,7 +215,10 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
   const int Mt = (Kdl + 15) >> 4;
   const int Rt = (Kl + 15) >> 4;
   const int K32l = (Kl + 31) & ~31;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the wave index is uniform over the wave: held in a scalar register, the
+  // per-wave tables (the GEMM 2 row ends) are read with scalar loads, and no
+  // vector-memory wait for them holds back the loads in flight
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lk = lane >> 4;
   const int e0 = blockIdx.x * NE;
   const int KB = K32 / 8;
