@@ -224,7 +224,9 @@ __global__ void __launch_bounds__(NT) k_elem_gemm(DevSpace S, OpCoef op, const T
   const int Mt = (Kdl + 15) >> 4;
   const int Rt = (Kl + 15) >> 4;
   const int Kpl = (Kl + 3) & ~3;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // wave-uniform in a scalar register: the per-wave tables (GEMM row starts
+  // and ends) are read with scalar loads, with no vector-memory wait
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lk = lane >> 4;
   const int e0 = blockIdx.x * NE;
   const T cst = (T)op.stiff;
