@@ -41,6 +41,12 @@ static const bool g_fused_alpha = [] {
   const char* v = getenv("TSEM_FUSED_ALPHA");
   return !(v && v[0] == '0');
 }();
+// beta formed inside the p update likewise (needs the in-update alpha, which
+// moves the latest r.z into place; TSEM_FUSED_BETA=0 keeps k_final_beta)
+static const bool g_fused_beta = [] {
+  const char* v = getenv("TSEM_FUSED_BETA");
+  return !(v && v[0] == '0');
+}();
 
 const char* tsem_last_error(void) { return g_err.c_str(); }
 
@@ -1041,16 +1047,17 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
   // x += alpha p, r += alpha q, z = P r; partials [r.z | |r|_1 | |r|_2^2] then
   // beta = r.z / r.z_old in one final reduction.  With pq_part, alpha is
   // formed from the p.q partials inside the update; else it is read from sc
-  auto update_and_precondition = [&](const double* pq_part) {
+  // finish = false leaves beta to k_p_update_beta
+  auto update_and_precondition = [&](const double* pq_part, bool finish) {
     launch_cg_update<T>(d, n, x, r, precond == 4 ? nullptr : z, p, q, sc, part, precond == 4 ? 0 : 1, pq_part,
                         st);
     if (precond == 4) {
       sp->pmg->vcycle<T>(r, z, smooth, st);
       launch_dot<T>(n, r, z, part, st);  // replaces the r.z partials
     }
-    launch_final_beta(part, nb, 1, sc, st);
+    if (finish) launch_final_beta(part, nb, 1, sc, st);
   };
-  update_and_precondition(nullptr);
+  update_and_precondition(nullptr, true);
   // p = -z
   launch_axpby<T>(n, T(-1), z, T(0), p, st);
   CgScalars hs{};
@@ -1062,20 +1069,21 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
   double res = resid(hs);
   int it = 0;
   static const bool trace = getenv("TSEM_PCG_TRACE") != nullptr;
-  if (trace) fprintf(stderr, "[tsem pcg] start res %.3e bnorm %.3e rz %.3e\n", res, bnorm, hs.rz);
+  if (trace) fprintf(stderr, "[tsem pcg] start res %.3e bnorm %.3e rz %.3e\n", res, bnorm, hs.rz_new);
   // the p.q partials go to the fourth quarter of the partial buffer, which
   // the update's own partials do not reach
   double* part_pq = part + 3 * (size_t)nb;
   auto iteration = [&]() {
     if (g_fused_alpha) {
       launch_apply<T>(d, op, p, q, yl, part_pq, st);
-      update_and_precondition(part_pq);
+      update_and_precondition(part_pq, !g_fused_beta);
     } else {
       launch_apply<T>(d, op, p, q, yl, part, st);
       launch_final_alpha(part, nb, sc, st);
-      update_and_precondition(nullptr);
+      update_and_precondition(nullptr, true);
     }
-    launch_p_update<T>(n, p, z, sc, st);
+    if (g_fused_alpha && g_fused_beta) launch_p_update_beta<T>(n, p, z, sc, part, st);
+    else launch_p_update<T>(n, p, z, sc, st);
   };
   auto& G = (sizeof(T) == 8) ? sp->graph64 : sp->graph32;
   double last_chunk = 0.0;  // host seconds of the previous chunk
@@ -1136,7 +1144,7 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
       auto refresh = [&](bool restart) {
         static const double zero = 0.0;
         HIPCHK(hipMemcpyAsync(&sc->alpha, &zero, sizeof(double), hipMemcpyHostToDevice, st));
-        update_and_precondition(nullptr);
+        update_and_precondition(nullptr, true);
         if (restart) launch_axpby<T>(n, T(-1), z, T(0), p, st);
         HIPCHK(hipMemcpyAsync(&hs, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -1147,7 +1155,7 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
     }
     if (trace)
       fprintf(stderr, "[tsem pcg%s] it %d res %.3e l1 %.3e l2 %.3e rz %.3e pq %.3e alpha %.3e beta %.3e\n",
-              sizeof(T) == 8 ? "64" : "32", it, res, hs.l1, std::sqrt(hs.l2), hs.rz, hs.pq, hs.alpha, hs.beta);
+              sizeof(T) == 8 ? "64" : "32", it, res, hs.l1, std::sqrt(hs.l2), hs.rz_new, hs.pq, hs.alpha, hs.beta);
     if (!std::isfinite(res)) break;
   }
   *iters = it;

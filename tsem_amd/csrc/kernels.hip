@@ -674,8 +674,10 @@ void launch_scalar_beta(CgScalars* sc, hipStream_t st) { hipLaunchKernelGGL(k_be
 // partial sums of r.z, |r|_1, |r|_2^2  (src/TSEMSolver.cpp:1846-1897).
 // With pq_part, every block first sums the nb partials of p.q itself, in the
 // order and with the code of k_final_alpha (so every block gets the same,
-// bitwise, alpha = rz / pq), and block 0 records pq and alpha; otherwise
-// alpha is read from sc.  pq_part must not overlap partial.
+// bitwise, alpha = rz_new / pq), and block 0 records pq and alpha and moves
+// rz_new into rz (rz_new is the latest r.z: k_final_beta sets both,
+// k_p_update_beta only rz_new); otherwise alpha is read from sc.  pq_part
+// must not overlap partial.
 template <typename T>
 __global__ void __launch_bounds__(BLK) k_cg_update(DevSpace S, int n, T* __restrict__ x, T* __restrict__ r,
                                                    T* __restrict__ z, const T* __restrict__ p,
@@ -689,10 +691,11 @@ __global__ void __launch_bounds__(BLK) k_cg_update(DevSpace S, int n, T* __restr
     double s = 0.0;
     for (int k = threadIdx.x; k < pq_nb; k += BLK) s += pq_part[k];
     const double pq = block_sum(s, red);
-    alpha_d = sc->rz / pq;
+    alpha_d = sc->rz_new / pq;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->pq = pq;
       sc->alpha = alpha_d;
+      sc->rz = sc->rz_new;  // the denominator of the next beta
     }
   } else {
     alpha_d = sc->alpha;
@@ -749,6 +752,47 @@ void launch_p_update(int n, T* p, const T* z, CgScalars* sc, hipStream_t st) {
 }
 template void launch_p_update<double>(int, double*, const double*, CgScalars*, hipStream_t);
 template void launch_p_update<float>(int, float*, const float*, CgScalars*, hipStream_t);
+
+// k_final_beta and p = beta p - z in one kernel: every block sums the r.z
+// partials [0, nb) itself, with the loop and tree of k_final_beta (so every
+// block gets the same beta, bitwise, as that kernel gives), and block 0 also
+// sums |r|_1 and |r|_2^2 and records rz_new, beta, l1 and l2.  sc->rz (the
+// previous r.z, read by every block) is not written here: the next CG update
+// with pq partials copies rz_new into it.
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_p_update_beta(int n, T* __restrict__ p, const T* __restrict__ z,
+                                                       CgScalars* __restrict__ sc,
+                                                       const double* __restrict__ partial, int nb) {
+  __shared__ double red[BLK];
+  double s0 = 0.0;
+  for (int b = threadIdx.x; b < nb; b += BLK) s0 += partial[b];
+  const double rz = block_sum(s0, red);
+  const double beta = rz / sc->rz;
+  if (blockIdx.x == 0) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int b = threadIdx.x; b < nb; b += BLK) {
+      s1 += partial[(size_t)nb + b];
+      s2 += partial[2 * (size_t)nb + b];
+    }
+    const double l1 = block_sum(s1, red);
+    const double l2 = block_sum(s2, red);
+    if (threadIdx.x == 0) {
+      sc->rz_new = rz;
+      sc->beta = beta;
+      sc->l1 = l1;
+      sc->l2 = l2;
+    }
+  }
+  const T bt = (T)beta;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) p[i] = bt * p[i] - z[i];
+}
+template <typename T>
+void launch_p_update_beta(int n, T* p, const T* z, CgScalars* sc, const double* partial, hipStream_t st) {
+  const int nb = reduce_blocks(n);
+  hipLaunchKernelGGL(k_p_update_beta<T>, dim3(nb), dim3(BLK), 0, st, n, p, z, sc, partial, nb);
+}
+template void launch_p_update_beta<double>(int, double*, const double*, CgScalars*, const double*, hipStream_t);
+template void launch_p_update_beta<float>(int, float*, const float*, CgScalars*, const double*, hipStream_t);
 
 template <typename T>
 __global__ void k_axpby(int n, T a, const T* __restrict__ x, T b, T* __restrict__ y) {

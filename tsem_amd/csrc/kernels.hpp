@@ -185,6 +185,10 @@ void launch_cg_update(const DevSpace& S, int n, T* x, T* r, T* z, const T* p, co
                       double* partial, int precond_jacobi, const double* pq_part, hipStream_t st);
 template <typename T>
 void launch_p_update(int n, T* p, const T* z, CgScalars* sc, hipStream_t st);
+// beta from the reduce_blocks(n) r.z partials (and |r|_1, |r|_2^2 recorded),
+// then p = beta p - z; sc->rz is left for the next pq-partials CG update
+template <typename T>
+void launch_p_update_beta(int n, T* p, const T* z, CgScalars* sc, const double* partial, hipStream_t st);
 void launch_scalar_alpha(CgScalars* sc, hipStream_t st);
 void launch_final_alpha(const double* partial, int nb, CgScalars* sc, hipStream_t st);
 void launch_final_beta(const double* partial, int nb, int with_rz, CgScalars* sc, hipStream_t st);
