@@ -686,6 +686,13 @@ __global__ void __launch_bounds__(BLK) k_cg_update(DevSpace S, int n, T* __restr
                                                    const double* __restrict__ pq_part, int pq_nb) {
   __shared__ double red[BLK];
   double a = 0, b = 0, c = 0;
+  const T* __restrict__ dg = (sizeof(T) == 8) ? (const T*)S.invdiag : (const T*)S.invdiag32;
+  // with the reduction first, the first element's loads go out before it,
+  // behind its latency
+  const int i0 = blockIdx.x * BLK + threadIdx.x;
+  const bool pre = pq_part && i0 < n;
+  const T x0 = pre ? x[i0] : T(0), p0 = pre ? p[i0] : T(0), q0 = pre ? q[i0] : T(0), r0 = pre ? r[i0] : T(0);
+  const T d0 = (pre && jac) ? dg[i0] : T(0);
   double alpha_d;
   if (pq_part) {
     double s = 0.0;
@@ -701,22 +708,22 @@ __global__ void __launch_bounds__(BLK) k_cg_update(DevSpace S, int n, T* __restr
     alpha_d = sc->alpha;
   }
   const T alpha = (T)alpha_d;
-  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
-    x[i] += alpha * p[i];
-    T ri = r[i] + alpha * q[i];
+  auto step = [&](int i, T xi, T pi, T qi, T rv, T di) {
+    x[i] = xi + alpha * pi;
+    const T ri = rv + alpha * qi;
     r[i] = ri;
-    T zi;
-    if (jac) {
-      if constexpr (sizeof(T) == 8) zi = ri * S.invdiag[i];
-      else zi = ri * S.invdiag32[i];
-    } else {
-      zi = ri;
-    }
+    const T zi = jac ? ri * di : ri;
     if (z) z[i] = zi;
     a += (double)ri * (double)zi;
     b += fabs((double)ri);
     c += (double)ri * (double)ri;
+  };
+  int i = i0;
+  if (pq_part) {
+    if (pre) step(i0, x0, p0, q0, r0, d0);
+    i += gridDim.x * BLK;
   }
+  for (; i < n; i += gridDim.x * BLK) step(i, x[i], p[i], q[i], r[i], jac ? dg[i] : T(0));
   a = block_sum(a, red);
   b = block_sum(b, red);
   c = block_sum(c, red);
@@ -764,6 +771,9 @@ __global__ void __launch_bounds__(BLK) k_p_update_beta(int n, T* __restrict__ p,
                                                        CgScalars* __restrict__ sc,
                                                        const double* __restrict__ partial, int nb) {
   __shared__ double red[BLK];
+  // the first element's loads go out before the reduction, behind its latency
+  const int i0 = blockIdx.x * BLK + threadIdx.x;
+  const T p0 = i0 < n ? p[i0] : T(0), z0 = i0 < n ? z[i0] : T(0);
   double s0 = 0.0;
   for (int b = threadIdx.x; b < nb; b += BLK) s0 += partial[b];
   const double rz = block_sum(s0, red);
@@ -784,7 +794,8 @@ __global__ void __launch_bounds__(BLK) k_p_update_beta(int n, T* __restrict__ p,
     }
   }
   const T bt = (T)beta;
-  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) p[i] = bt * p[i] - z[i];
+  if (i0 < n) p[i0] = bt * p0 - z0;
+  for (int i = i0 + gridDim.x * BLK; i < n; i += gridDim.x * BLK) p[i] = bt * p[i] - z[i];
 }
 template <typename T>
 void launch_p_update_beta(int n, T* p, const T* z, CgScalars* sc, const double* partial, hipStream_t st) {
