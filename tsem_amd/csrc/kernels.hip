@@ -387,9 +387,10 @@ __global__ void __launch_bounds__(BLK) k_scatter_slab(DevSpace S, const T* __res
   const int q0 = (int)((long long)xcd * ngr / nx), q1 = (int)((long long)(xcd + 1) * ngr / nx);
   const int n1 = S.sl_n1, n2 = S.sl_n2, cm = S.sl_chunk_max;
   const bool need_x = partial || constrained;
-  // span bounds of the step about to run; the next step's are loaded while
-  // this one's values are in flight.  The table is padded with empty chunks
-  // to whole groups, so a step's bounds are read without a test
+  // span bounds of the step about to run; each step loads the next step's
+  // (the compiler places that load at the end of the step).  The table is
+  // padded with empty chunks to whole groups, so a step's bounds are one
+  // scalar load without a test
   int lo[SP], len[SP];
   auto bounds = [&](int q) {
     const int4* cb = reinterpret_cast<const int4*>(S.sl_chunk) + (size_t)q * (SP / 2);
