@@ -23,21 +23,48 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
-def cpu_baseline(n=8, p=6, seed_mesh=2026, seed_prob=1234, qorder=None):
-    """Oracle (CPU restatement of the reference: assembled CSR + Jacobi PCG,
-    one thread) on the same workload family at a bounded size."""
+def host_threads():
+    """Host threads this process may use: the OpenMP setting when given,
+    else the CPUs of its affinity mask, at most 16 (a one-GPU box's share)."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = int(env) if env and env.isdigit() else len(os.sched_getaffinity(0))
+    return max(1, min(16, n))
+
+
+def cpu_baseline(n=8, p=4, seed_prob=1234, repeats=5):
+    """The oracle (the CPU restatement of the reference path: assembled FP64
+    CSR with diagonal-first rows, solve_PCG of src/TSEMSolver.cpp:1775-1930)
+    on the reference's own CPU case: the n = 8 Kuhn cube (3,072 tets), p = 4,
+    mixed BC, manufactured u, to rel-res 1e-10 from x = 0.  Four variants:
+    flag 1 (Jacobi, the reference default) and flag 4 (Gauss-Seidel
+    p-multigrid, s = 2, one V-cycle), each on one thread (the reference is
+    serial) and on all host threads (OpenMP products and vector sweeps; the
+    Gauss-Seidel sweeps stay sequential).  Median solve time of `repeats`
+    runs per variant (p-MG setup timed apart).  `value` is the fastest
+    variant's DOFs/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import harness
     from tsem_amd import mesh, problems
     prob = problems.Manufactured(seed_prob)
-    xyz, tet = mesh.kuhn_cube(n, jitter=0.1, seed=seed_mesh)
-    S, b = harness.setup_poisson(p, xyz, tet, prob, qorder=qorder or p + 4)
+    xyz, tet = mesh.kuhn_cube(n)
     t0 = time.perf_counter()
-    x, st = S.pcg(b, tol=1e-10, criterion=1, precond=1)
-    dt = time.perf_counter() - t0
-    return dict(value=S.ndof / dt, unit="DOFs/s", cores=1, kind="port",
-                sample="n=%d (%d tets), p=%d, %d DOFs, assembled-CSR Jacobi PCG to rel-res 1e-10, %d iterations, %.2f s"
-                       % (n, len(tet), p, S.ndof, st["iterations"], dt))
+    S, b = harness.setup_poisson(p, xyz, tet, prob, qorder=p + 4)
+    setup = time.perf_counter() - t0
+    nth = host_threads()
+    variants = {}
+    for name, pc in (("jacobi", 1), ("pmg_gs", 4)):
+        for th in sorted({1, nth}):
+            runs = [S.pcg(b, tol=1e-10, criterion=1, precond=pc, smooth=2, threads=th)[1] for _ in range(repeats)]
+            t = float(np.median([r["solve_s"] for r in runs]))
+            variants["%s_%dt" % (name, th)] = {"dofs_per_s": S.ndof / t, "solve_s": t, "threads": th,
+                                                "iterations": runs[0]["iterations"], "relres": runs[0]["relres"],
+                                                "pmg_setup_s": runs[0]["setup_s"]}
+    best = max(variants, key=lambda k: variants[k]["dofs_per_s"])
+    return dict(value=variants[best]["dofs_per_s"], unit="DOFs/s", cores=variants[best]["threads"], kind="port",
+                variant=best, host_threads=nth, variants=variants, assembly_setup_s=setup,
+                sample="n=8 Kuhn cube (3072 tets), p=4, %d DOFs, mixed BC, PCG to rel-res 1e-10: flag 1 (Jacobi) "
+                       "and flag 4 (Gauss-Seidel p-MG), 1 and %d threads, median of %d solves each"
+                       % (S.ndof, nth, repeats))
 
 
 def profile_traffic(kernel, n, p):
@@ -212,6 +239,8 @@ def main():
     b = api.poisson_setup(sp, prob)
     log("rhs + bc %.1f s" % (time.perf_counter() - t1))
 
+    last = {}
+
     def solve(mixed, precond):
         x = sp.zeros()
         torch.cuda.synchronize()
@@ -222,6 +251,7 @@ def main():
         dt = time.perf_counter() - t
         if not st["converged"] or st["relres"] > 1e-10:
             raise RuntimeError("solve did not reach rel-res 1e-10: %s" % st)
+        last[(mixed, precond)] = x
         return dt, st
 
     from tsem_amd import lib as tl
@@ -241,6 +271,12 @@ def main():
     tl.check(tl.load().tsem_timing(1))
     solve(True, args.precond)
     tl.check(tl.load().tsem_timing_read(tl.dptr(kt)))
+    tl.check(tl.load().tsem_timing(0))
+    # the same for one FP64 Jacobi solve (full-order applies only)
+    kt64 = np.zeros(12)
+    tl.check(tl.load().tsem_timing(1))
+    solve(False, 1)
+    tl.check(tl.load().tsem_timing_read(tl.dptr(kt64)))
     tl.check(tl.load().tsem_timing(0))
     # FP64 drop-in: the reference's PCG on the same system, matrix-free FP64,
     # with Jacobi (the reference default) and with p-multigrid
@@ -308,14 +344,35 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": kd["gbs"], "peak": 8000.0, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"] if roof["achieved"] else None
     roof["traffic"] = profile_traffic(dom, args.n, args.p)
-    # the whole mixed solve against HBM: algorithmic bytes of one FP32
-    # iteration (the three apply kernels -- the element kernel reads the split
-    # tile and Ghat, writes E K results and with slabs reads 64 B of entity
-    # offsets per element --, k_cg_update: x, r, p, q, 1/diag read and x, r, z
-    # written; k_p_update: p, z read, p written) x iterations / time
-    it_bytes = sum(v.get("algorithmic_bytes", 0.0) for v in kernels.values()) \
-        + (4.0 * E * K32 + 24.0 * E + 4.0 * E * K + (64.0 * E if slab else 0.0)) + 32.0 * N + 12.0 * N
+    # the whole solve against HBM, by the algorithmic bytes of one Jacobi PCG
+    # iteration (s bytes per value): the operator reads p and writes A p
+    # (2 N s), reads the geometric factors (6 E s) and one int32 gather slot
+    # per local coefficient (4 E K) plus 12 B per extra face-combination
+    # term of the gather map (F_x = map terms - E K); the fused CG updates
+    # move 11 N s (x += alpha p: 3, r += alpha A p with z = D^-1 r and the
+    # fused r.z: 5, p = z + beta p: 3).  Intermediates this design writes and
+    # reads back (the fp16 split tile, the entity slab) are not algorithmic
+    # bytes; they are reported beside it.
+    def b_iter(sz):
+        fx = max(0.0, nnz_sc - E * K)
+        return 2.0 * N * sz + 6.0 * E * sz + 4.0 * E * K + 12.0 * fx + 11.0 * N * sz
+
+    it_bytes = b_iter(4.0)
+    inter_bytes = 2 * (4.0 * E * K32 + 4.0 * E) + (2 * 4.0 * E * K if slab else 2 * 4.0 * E * K)
     n_it = stats[-1]["iterations"]
+    # per-kernel FP64 timing (the drop-in's full-order applies)
+    k64 = {}
+    for i, name in enumerate(("gather64", "element64", "scatter64")):
+        tot, cnt = kt64[2 * i], kt64[2 * i + 1]
+        k64[name] = {"avg_ms": tot / cnt if cnt else None, "launches": int(cnt), "total_ms": tot}
+    if k64["element64"]["avg_ms"]:
+        k64["element64"]["tflops"] = 12.0 * Kd * K * E / (k64["element64"]["avg_ms"] * 1e-3) / 1e12
+        k64["element64"]["frac_of_fp64_peak"] = k64["element64"]["tflops"] / 78.6
+    # each result's error against the FP64 solve of the same system (both at
+    # rel-res 1e-10)
+    xm, x64 = last.get((True, args.precond)), last.get((False, 1))
+    err_vs_fp64 = float(torch.linalg.norm(xm - x64) / torch.linalg.norm(x64)) if xm is not None and x64 is not None \
+        else None
     value = sp.ndof / mean
     fp64_value = max(v["value"] for v in fp64.values()) if fp64 else None
     out = {
@@ -346,8 +403,14 @@ def main():
         "roofline": roof,
         "solve_hbm": {"bytes_per_fp32_iteration": it_bytes, "iterations": n_it,
                       "achieved_GBps": it_bytes * n_it / mean / 1e9,
-                      "frac": it_bytes * n_it / mean / 8e12},
+                      "frac": it_bytes * n_it / mean / 8e12,
+                      "intermediate_bytes_per_iteration": inter_bytes,
+                      "fp64_jacobi": ({"bytes_per_iteration": b_iter(8.0), "iterations": fp64["jacobi"]["iterations"],
+                                       "frac": b_iter(8.0) * fp64["jacobi"]["iterations"] / (fp64["jacobi"]["ms"] * 1e-3)
+                                       / 8e12} if "jacobi" in fp64 else None)},
+        "mixed_vs_fp64_rel_diff": err_vs_fp64,
         "kernels": kernels,
+        "kernels_fp64": k64,
         "cpu_baseline": None,
     }
     for key, skip, fn in (("eigen", args.no_eigen, lambda: eigen_section(ctx, log)),
@@ -362,8 +425,8 @@ def main():
             log("%s failed: %r" % (key, e))
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(p=args.p)
-            out["cpu_baseline"]["cores"] = 1
+            out["cpu_baseline"] = cpu_baseline()
+            log("cpu baseline %s" % out["cpu_baseline"])
         except Exception as e:  # reported, not fatal
             log("cpu baseline failed: %r" % (e,))
     print(json.dumps(out), flush=True)

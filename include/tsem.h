@@ -374,8 +374,9 @@ typedef struct {
 } tsem_serve_opts;
 
 typedef struct {
-  int64_t topology;       /* caller's key: requests with one nonzero key share connectivity, order,
-                             quadrature and boundary labels, and reuse a cached space; 0: none */
+  int64_t topology;       /* caller's key: requests with one nonzero key share connectivity, order
+                             and quadrature, and reuse a cached space (each request's own labels and
+                             Dirichlet / Neumann parts are applied to it); 0: none */
   int M, qorder;          /* order; quadrature order (0: M + 3) */
   int nv, nt;
   const double* xyz;      /* host, nv x 3 (copied by submit) */

@@ -58,6 +58,8 @@ def lib():
         L.or_sum_dof.argtypes = [vp, ip]
         L.or_pcg.restype = C.c_int
         L.or_pcg.argtypes = [vp, dp, dp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, ip, C.c_int, dp]
+        L.or_pcg_mt.restype = C.c_int
+        L.or_pcg_mt.argtypes = [vp, dp, dp, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, ip, C.c_int, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -239,10 +241,15 @@ class OracleSpace:
     def zero_bc(self, flag, on_matrix=True, vec=None):
         lib().or_zero_bc(self.h, flag, 1 if on_matrix else 0, _d(vec) if vec is not None else None)
 
-    def pcg(self, b, x0=None, tol=1e-10, max_step=100000, precond=1, ite_step=1, smooth=2, criterion=1):
+    def pcg(self, b, x0=None, tol=1e-10, max_step=100000, precond=1, ite_step=1, smooth=2, criterion=1, threads=1):
+        """solve_PCG on the assembled CSR; threads > 1 runs the products and
+        vector sweeps on that many OpenMP threads (Gauss-Seidel stays
+        sequential).  Stats carry the p-MG setup and solve seconds."""
         x = np.zeros(self.ndof) if x0 is None else np.array(x0, dtype=np.float64)
         b = np.ascontiguousarray(b, dtype=np.float64)
         sd = self.sum_dof() if precond == 4 else np.zeros(1, np.int32)
-        st = np.zeros(3)
-        rc = lib().or_pcg(self.h, _d(x), _d(b), tol, max_step, precond, ite_step, smooth, _i(sd), criterion, _d(st))
-        return x, dict(converged=(rc == 0), iterations=int(st[0]), l1=st[1], relres=st[2])
+        st = np.zeros(5)
+        rc = lib().or_pcg_mt(self.h, _d(x), _d(b), tol, max_step, precond, ite_step, smooth, _i(sd), criterion,
+                             int(threads), _d(st))
+        return x, dict(converged=(rc == 0), iterations=int(st[0]), l1=st[1], relres=st[2], setup_s=st[3],
+                       solve_s=st[4])

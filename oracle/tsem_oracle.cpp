@@ -36,6 +36,7 @@
 // (pinned only through the solution it converges to).
 #include "oracle_basis.hpp"
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -90,7 +91,8 @@ struct CSR {
     if (it == e || *it != c) return -1;
     return it - col.begin();
   }
-  void vmult(const double* x, double* y) const {
+  void vmult(const double* x, double* y, int nthreads = 1) const {
+#pragma omp parallel for schedule(static) num_threads(nthreads) if (nthreads > 1)
     for (int r = 0; r < n; ++r) {
       double s = 0;
       for (long k = rs[r]; k < rs[r + 1]; ++k) s += v[k] * x[col[k]];
@@ -1069,8 +1071,9 @@ static void init_pmg(PMG& P, const CSR& A, int p, const int* sum_dof) {
   }
 }
 
-// ite_PMG (src/TSEMSolver.cpp:1119-1184)
-static void ite_pmg(const PMG& P, double* x, const double* r, int ite_step, int smooth) {
+// ite_PMG (src/TSEMSolver.cpp:1119-1184); the Gauss-Seidel sweeps are
+// sequential as in the reference, the level residuals' products use nthreads
+static void ite_pmg(const PMG& P, double* x, const double* r, int ite_step, int smooth, int nthreads = 1) {
   int np = P.n_project;
   std::vector<std::vector<double>> px(np + 1), pr(np + 1);
   px[0].assign(x, x + P.level[0].n);
@@ -1080,7 +1083,7 @@ static void ite_pmg(const PMG& P, double* x, const double* r, int ite_step, int 
     for (int i = 0; i < np; ++i) {
       gs_forward(P.level[i], px[i].data(), pr[i].data(), smooth);
       std::vector<double> v0(P.level[i].n), v1(pr[i]);
-      P.level[i].vmult(px[i].data(), v0.data());
+      P.level[i].vmult(px[i].data(), v0.data(), nthreads);
       for (int k = 0; k < P.level[i].n; ++k) v1[k] -= v0[k];
       for (int k = 0; k < P.level[i].n; ++k)
         if (P.core_at[i][k] >= 0) pr[i + 1][P.core_at[i][k]] = v1[k];
@@ -1104,21 +1107,39 @@ extern "C" {
 // precond: 1 Jacobi, 4 p-MG (needs sum_dof, p, ite_step, smooth).
 // criterion: 0 = reference (l1 norm of recursive residual < tol);
 //            1 = relative ||r||_2/||b||_2 < tol on the recursive residual.
-// stats out: [iterations, final l1 residual, final rel l2 residual]
-int or_pcg(void* p, double* x, const double* b, double tol, int max_step, int precond, int ite_step, int smooth,
-           const int* sum_dof, int criterion, double* stats) {
+// nthreads > 1: the products and vector sweeps run on that many OpenMP
+// threads (the sums then run in another order; the Gauss-Seidel sweeps stay
+// sequential, as in the reference); 1 is the sequential reference form.
+// stats out: [iterations, final l1 residual, final rel l2 residual,
+//             p-MG setup seconds, solve seconds] (steady clock)
+int or_pcg_mt(void* p, double* x, const double* b, double tol, int max_step, int precond, int ite_step, int smooth,
+              const int* sum_dof, int criterion, int nthreads, double* stats) {
   OrHandle* h = (OrHandle*)p;
   const CSR& A = *h->A;
-  int n = A.n;
+  const int n = A.n;
+  const int nt = nthreads > 1 ? nthreads : 1;
+  auto t0 = std::chrono::steady_clock::now();
   PMG P;
   if (precond == 4) init_pmg(P, A, h->s.M, sum_dof);
+  auto t1 = std::chrono::steady_clock::now();
   std::vector<double> tmp(n), rk(n), yk(x, x + n), pk(b, b + n);
-  A.vmult(x, rk.data());
+  A.vmult(x, rk.data(), nt);
   for (int i = 0; i < n; ++i) rk[i] -= b[i];
   auto precondition = [&](std::vector<double>& y) {
-    if (precond == 1) for (int i = 0; i < n; ++i) y[i] = rk[i] / A.diag(i);
-    else if (precond == 4) ite_pmg(P, y.data(), rk.data(), ite_step, smooth);
-    else y = rk;
+    if (precond == 1) {
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
+      for (int i = 0; i < n; ++i) y[i] = rk[i] / A.diag(i);
+    } else if (precond == 4) {
+      ite_pmg(P, y.data(), rk.data(), ite_step, smooth, nt);
+    } else {
+      y = rk;
+    }
+  };
+  auto dot = [&](const std::vector<double>& a, const std::vector<double>& c) {
+    double s = 0;
+#pragma omp parallel for schedule(static) reduction(+ : s) num_threads(nt) if (nt > 1)
+    for (int i = 0; i < n; ++i) s += a[i] * c[i];
+    return s;
   };
   precondition(yk);
   for (int i = 0; i < n; ++i) pk[i] = -yk[i];
@@ -1126,36 +1147,52 @@ int or_pcg(void* p, double* x, const double* b, double tol, int max_step, int pr
   for (int i = 0; i < n; ++i) bnorm += b[i] * b[i];
   bnorm = std::sqrt(bnorm);
   auto resid = [&]() {
-    if (criterion == 0) { double s = 0; for (int i = 0; i < n; ++i) s += std::fabs(rk[i]); return s; }
-    double s = 0; for (int i = 0; i < n; ++i) s += rk[i] * rk[i];
-    return std::sqrt(s) / bnorm;
+    double s = 0;
+    if (criterion == 0) {
+#pragma omp parallel for schedule(static) reduction(+ : s) num_threads(nt) if (nt > 1)
+      for (int i = 0; i < n; ++i) s += std::fabs(rk[i]);
+      return s;
+    }
+    return std::sqrt(dot(rk, rk)) / bnorm;
   };
   double residual = resid();
   int iter = 0;
-  double rty = 0;
-  for (int i = 0; i < n; ++i) rty += rk[i] * yk[i];
+  double rty = dot(rk, yk);
   while (residual >= tol) {
-    A.vmult(pk.data(), tmp.data());
-    double pap = 0;
-    for (int i = 0; i < n; ++i) pap += pk[i] * tmp[i];
+    A.vmult(pk.data(), tmp.data(), nt);
+    double pap = dot(pk, tmp);
     double alpha = rty / pap;
-    for (int i = 0; i < n; ++i) x[i] += alpha * pk[i];
-    for (int i = 0; i < n; ++i) rk[i] += alpha * tmp[i];
-    std::fill(yk.begin(), yk.end(), 0.0);
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
+    for (int i = 0; i < n; ++i) {
+      x[i] += alpha * pk[i];
+      rk[i] += alpha * tmp[i];
+      yk[i] = 0.0;
+    }
     precondition(yk);
-    double rty1 = 0;
-    for (int i = 0; i < n; ++i) rty1 += rk[i] * yk[i];
+    double rty1 = dot(rk, yk);
     double beta = rty1 / rty;
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
     for (int i = 0; i < n; ++i) pk[i] = beta * pk[i] - yk[i];
     residual = resid();
     iter++;
     if (iter > max_step) break;
     rty = rty1;
   }
+  auto t2 = std::chrono::steady_clock::now();
   double l1 = 0, l2 = 0;
   for (int i = 0; i < n; ++i) { l1 += std::fabs(rk[i]); l2 += rk[i] * rk[i]; }
   stats[0] = iter; stats[1] = l1; stats[2] = std::sqrt(l2) / bnorm;
+  stats[3] = std::chrono::duration<double>(t1 - t0).count();
+  stats[4] = std::chrono::duration<double>(t2 - t1).count();
   return residual < tol ? 0 : 1;
+}
+
+int or_pcg(void* p, double* x, const double* b, double tol, int max_step, int precond, int ite_step, int smooth,
+           const int* sum_dof, int criterion, double* stats) {
+  double st[5];
+  int rc = or_pcg_mt(p, x, b, tol, max_step, precond, ite_step, smooth, sum_dof, criterion, 1, st);
+  for (int k = 0; k < 3; ++k) stats[k] = st[k];
+  return rc;
 }
 
 }  // extern "C"

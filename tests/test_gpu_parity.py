@@ -1,9 +1,9 @@
 """GPU parity of the HIP path against the CPU oracle (same seeded meshes).
 
 Bars: gather and scatter bit-exact (integer index work and fixed-order FP64
-sums); operator apply within 1e-13 relative (FP64) / 1e-5 (FP32); solves
-within 1e-10 relative of the oracle's solution when both are driven to a
-relative residual of 1e-13.
+sums); operator apply within 1e-13 (FP64) / 1e-6 (FP32 kernels; 2e-5 for the
+fp16-split one, on the FP32-rounded input) of the largest output entry; solves within 1e-10 relative of the
+oracle's solution when both are driven to a relative residual of 1e-13.
 """
 import numpy as np
 import pytest
@@ -59,7 +59,15 @@ def test_scatter_bit_exact(pair):
     assert np.array_equal(y, yp)
 
 
-@pytest.mark.parametrize("path", [0, 1], ids=["mfma", "scalar"])
+# FP32 bound per element kernel: 1e-6 of max |y| for the exact FP32 kernels
+# (scalar, FP32 MFMA); the fp16-split MFMA kernel (the default) carries each
+# operand as two fp16 halves and drops the lo*lo product, about 2^-22 per
+# product against FP32's 2^-24, and the stiffness rows cancel: measured
+# 9.9e-6 (p = 4) and 3.2e-6 (p = 6) of max |y| for random input
+FP32_BOUND = {0: 2e-5, 1: 1e-6, 2: 1e-6}
+
+
+@pytest.mark.parametrize("path", [0, 1, 2], ids=["mfma_fp16split", "scalar", "mfma_fp32"])
 @pytest.mark.parametrize("coef", [(1.0, 0.0), (0.0, 1.0), (0.5, -2.0)])
 def test_apply_matches_assembled(pair, coef, path):
     import torch
@@ -74,8 +82,15 @@ def test_apply_matches_assembled(pair, coef, path):
         y64 = psp.apply(torch.from_numpy(x).cuda(), stiff=coef[0], mass=coef[1]).cpu().numpy()
         scale = np.abs(y_ref).max()
         assert np.abs(y64 - y_ref).max() <= 1e-13 * scale
-        y32 = psp.apply(torch.from_numpy(x).float().cuda(), stiff=coef[0], mass=coef[1]).double().cpu().numpy()
-        assert np.abs(y32 - y_ref).max() <= 2e-5 * scale
+        # FP32: against the exact operator on the FP32-rounded input, so the
+        # bound measures the kernel's arithmetic, not the input rounding
+        x32 = x.astype(np.float32)
+        y_ref32 = osp.vmult(x32.astype(np.float64))
+        y32 = psp.apply(torch.from_numpy(x32).cuda(), stiff=coef[0], mass=coef[1]).double().cpu().numpy()
+        err32 = np.abs(y32 - y_ref32).max() / np.abs(y_ref32).max()
+        print("[result] FP32 apply M %d coef %s path %d: max error %.2e of max |y|" % (psp.M, coef, path, err32),
+              flush=True)
+        assert err32 <= FP32_BOUND[path]
     finally:
         lib.check(lib.load().tsem_set_apply_path(0))
 
@@ -154,11 +169,13 @@ def test_mixed_precision_reaches_fp64_accuracy(ctx, precond, mode, inner_tol):
     xyz, tet = mesh.kuhn_cube(3, jitter=0.1, seed=9)
     psp = api.Space(ctx, M, xyz, tet, qorder=M + 4)
     b = api.poisson_setup(psp, prob)
-    x64, s64 = psp.pcg(b, tol=1e-12, criterion=1, precond=precond)
-    xm, sm = psp.pcg(b, tol=1e-12, criterion=1, precond=precond, mixed=mode, inner_tol=inner_tol, check_every=5)
-    assert s64["converged"] and sm["converged"] and sm["relres"] <= 1e-12, sm
+    x64, s64 = psp.pcg(b, tol=1e-13, criterion=1, precond=precond)
+    xm, sm = psp.pcg(b, tol=1e-13, criterion=1, precond=precond, mixed=mode, inner_tol=inner_tol, check_every=5)
+    assert s64["converged"] and sm["converged"] and sm["relres"] <= 1e-13, sm
     a, c = x64.cpu().numpy(), xm.cpu().numpy()
-    assert np.linalg.norm(a - c) <= 1e-9 * np.linalg.norm(a)
+    print("[result] mixed %d precond %d: %.2e" % (mode, precond, np.linalg.norm(a - c) / np.linalg.norm(a)),
+          flush=True)
+    assert np.linalg.norm(a - c) <= 1e-10 * np.linalg.norm(a)
     if mode == 1 and (precond == 1 or inner_tol >= 1e-2):
         assert sm["outer"] >= 1 and sm["iterations"] <= 1.15 * s64["iterations"] + 10, (sm, s64)
 
@@ -220,3 +237,38 @@ def test_graph_replays_match_eager(ctx, precond):
         x0, i0 = res[(0, mixed)][0]
         for x1, i1 in res[(1, mixed)]:
             assert i1 == i0 and np.array_equal(x1, x0)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_graphs_follow_pmg_rebuilds(ctx, mixed):
+    """A captured PCG chunk is never replayed after the p-multigrid levels it
+    captured were rebuilt: p-MG solve, then a new geometry and new boundary
+    data on the same space (both drop the levels), then a p-MG solve again.
+    With graphs on, both solves give the iterates of eager launches, bit
+    for bit, and the second one the solution of a fresh space."""
+    from tsem_amd import lib
+    _say("graphs after rebuild", mixed)
+    prob = problems.Manufactured(1234)
+    xyz, tet = mesh.kuhn_cube(3, jitter=0.1, seed=13)
+    xyz2, _ = mesh.kuhn_cube(3, jitter=0.1, seed=14)
+    out = {}
+    try:
+        for g in (0, 1):
+            lib.check(lib.load().tsem_set_graphs(g))
+            psp = api.Space(ctx, 5, xyz, tet, qorder=8)
+            b = api.poisson_setup(psp, prob)
+            x1, s1 = psp.pcg(b, tol=1e-11, precond=4, mixed=mixed, check_every=7)
+            psp.set_geometry(xyz2)
+            b2 = api.poisson_setup(psp, prob)
+            x2, s2 = psp.pcg(b2, tol=1e-11, precond=4, mixed=mixed, check_every=7)
+            out[g] = (x1.cpu().numpy(), x2.cpu().numpy(), s1["iterations"], s2["iterations"])
+            psp.close()
+    finally:
+        lib.check(lib.load().tsem_set_graphs(1))
+    assert out[0][2:] == out[1][2:]
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    fresh = api.Space(ctx, 5, xyz2, tet, qorder=8)
+    bf = api.poisson_setup(fresh, prob)
+    xf, _ = fresh.pcg(bf, tol=1e-11, precond=4, mixed=mixed, check_every=7)
+    assert np.linalg.norm(xf.cpu().numpy() - out[1][1]) <= 1e-9 * np.linalg.norm(xf.cpu().numpy())
+    fresh.close()

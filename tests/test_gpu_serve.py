@@ -123,6 +123,31 @@ def test_native_server_reuses_cached_space(ctx):
         assert np.abs(x - ref).max() <= 1e-9 * np.abs(ref).max()
 
 
+def test_native_server_reapplies_marks_on_reuse(ctx):
+    """Two requests with one topology key but different boundary parts
+    (Dirichlet on x = 0 then on every face): the second reuses the cached
+    space and is solved with its own marks, as a fresh space would be."""
+    from tsem_amd import api, mesh, problems
+    _say("native serve marks")
+    srv = serve.NativeServer(slots=1)
+    xyz, tet = mesh.kuhn_cube(4, jitter=0.1, seed=21)
+    prob = problems.Manufactured(21)
+    lab = mesh.cube_plane_marks(xyz)
+    a = srv.submit(xyz, tet, 4, prob, labels=lab, topology=9)
+    _, xa, ra = srv.poll(a)
+    b = srv.submit(xyz, tet, 4, prob, labels=lab, dirichlet=0b111111, neumann=0, topology=9)
+    _, xb, rb = srv.poll(b)
+    srv.close()
+    assert ra["built"] == 1 and rb["built"] == 0 and rb["relres"] <= 1e-10
+    for x, dp, nb in ((xa, 0b000001, 0b111110), (xb, 0b111111, 0)):
+        sp = api.Space(ctx, 4, xyz, tet, qorder=7)
+        bb = api.poisson_setup(sp, prob, dirichlet_planes=dp, neumann_planes=nb)
+        ref, _ = sp.pcg(bb, tol=1e-10, criterion=1, mixed=True, check_every=10)
+        ref = ref.cpu().numpy()
+        sp.close()
+        assert np.abs(x - ref).max() <= 1e-9 * np.abs(ref).max(), dp
+
+
 def test_native_server_reports_callback_failure(ctx):
     """A load callback that fails ends that request with TSEM_EINVAL and its
     message; the server goes on with the next request."""

@@ -224,6 +224,65 @@ def test_dof_count_and_layout():
         assert sd.min() == 1 and sd.max() == M
 
 
+def _fitted_degrees(s, M):
+    """Polynomial degree of each reference mode, found by least-squares fits
+    against the monomials of degree d = 0, 1, ... at random points of the
+    reference tetrahedron (independent of any numbering)."""
+    rng = np.random.default_rng(0)
+    pts = rng.dirichlet(np.ones(4), size=1000)[:, 1:]
+    v, _ = s.basis_at(pts)
+    c = 4 * pts - 1  # centred monomials: better conditioned fits
+    deg = np.full(v.shape[1], -1)
+    for d in range(M + 1):
+        A = np.stack([c[:, 0] ** a * c[:, 1] ** b * c[:, 2] ** k
+                      for a in range(d + 1) for b in range(d + 1 - a) for k in range(d + 1 - a - b)], 1)
+        coef = np.linalg.lstsq(A, v, rcond=None)[0]
+        # a mode of degree d misses P_{d-1} by O(1); 1e-8 only absorbs the fit's rounding
+        fits = np.abs(A @ coef - v).max(0) <= 1e-8 * np.abs(v).max(0)
+        deg[(deg < 0) & fits] = d
+    assert (deg >= 0).all()
+    return np.maximum(deg, 1)  # vertex modes (P1 hats) count as degree 1
+
+
+@pytest.mark.parametrize("n,M", [(2, 4), (3, 6), (2, 8)])
+def test_sum_dof_matches_polynomial_degree(n, M):
+    """calc_sum_dof (src/Miscellaneous.cpp:557-587) against what the
+    mathematics fixes: the DOFs of degree <= k span the order-k space, which
+    has (k n + 1)^3 DOFs on n^3 Kuhn cells, for every k; and every gather term
+    of a local mode of (numerically fitted) degree d lands on a DOF of degree
+    d (modes of one degree are re-expressed among themselves).  A wrong face
+    or interior offset, or a swapped degree, fails one of them."""
+    xyz, tet = mesh.kuhn_cube(n, jitter=0.1, seed=n + M)
+    s = O.OracleSpace(M, xyz, tet)
+    sd = s.sum_dof()
+    for k in range(1, M + 1):
+        assert np.count_nonzero(sd <= k) == (k * n + 1) ** 3, k
+    deg = _fitted_degrees(s, M)
+    off, ind, val = s.maps()
+    K = len(deg)
+    assert len(off) == s.nt * K + 1
+    term_deg = np.repeat(np.tile(deg, s.nt), np.diff(off))
+    assert np.array_equal(sd[ind], term_deg)
+
+
+def test_l2_error_scale_closed_form():
+    """harness.l2_error against closed forms on the (jittered) unit cube:
+    ||0 - u|| for u = x is 1/sqrt(3), for u = 1 + x y z it is
+    sqrt(1 + 2/8 + 1/27) (exact quadrature, volumes summing to 1)."""
+
+    class U:
+        def __init__(self, f):
+            self.u = f
+
+    xyz, tet = mesh.kuhn_cube(2, jitter=0.1, seed=6)
+    s = O.OracleSpace(3, xyz, tet, qorder=6)
+    zero = np.zeros(s.ndof)
+    e1 = harness.l2_error(s, zero, U(lambda p: p[..., 0]))
+    assert abs(e1 - 1 / np.sqrt(3)) <= 1e-13
+    e2 = harness.l2_error(s, zero, U(lambda p: 1 + p[..., 0] * p[..., 1] * p[..., 2]))
+    assert abs(e2 - np.sqrt(1 + 2 / 8 + 1 / 27)) <= 1e-13
+
+
 @pytest.mark.parametrize("M", [3, 4, 6])
 def test_cubic_solution_is_exact(M):
     xyz, tet = mesh.kuhn_cube(2, jitter=0.1, seed=4)

@@ -892,7 +892,7 @@ int tsem_dirichlet(tsem_space* sp, const double* uv, const double* ue, const dou
   upload_mask(sp, st);
   HIPCHK(hipMemcpyAsync(b_dev, b.data(), S.ndof * 8, hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
-  sp->pmg.reset();
+  sp->drop_pmg();
   API_END
 }
 
@@ -909,7 +909,7 @@ int tsem_zero_bc(tsem_space* sp, int flag) {
   }
   upload_mask(sp, sp->st());
   HIPCHK(hipStreamSynchronize(sp->st()));
-  sp->pmg.reset();
+  sp->drop_pmg();
   API_END
 }
 
@@ -1095,7 +1095,7 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
     if (g_pcg_graphs && !timing_on() && it + nrun <= max_iter + 1) {
       // the chunk of nrun iterations as one captured graph, replayed while
       // the pointers and parameters it captured stay the same
-      key = {x, r, z, p, q, yl, part, sc, sp->pmg.get(), (const void*)(intptr_t)precond,
+      key = {x, r, z, p, q, yl, part, sc, (const void*)(uintptr_t)sp->state_gen, (const void*)(intptr_t)precond,
              (const void*)(intptr_t)smooth, (const void*)(intptr_t)nrun, (const void*)(intptr_t)apply_path(),
              (const void*)(intptr_t)slab_enabled()};
       if ((!G.exec || G.key != key) && G.failed != key) {

@@ -111,6 +111,17 @@ struct tsem_space {
     ~PcgGraph() { reset(); }
   } graph64, graph32;
   std::unique_ptr<Pmg> pmg;
+  // bumped whenever the operator state a captured PCG chunk depends on
+  // changes (p-multigrid levels, eliminations, geometry): part of the graph
+  // key, so no graph outlives the buffers and coefficients it captured
+  uint64_t state_gen = 0;
+  // drop the p-multigrid levels and every graph that captured them
+  void drop_pmg() {
+    pmg.reset();
+    graph64.reset();
+    graph32.reset();
+    ++state_gen;
+  }
   bool rhs_open = false;
   hipStream_t st() const {
     if (!ctx) throw std::logic_error("space was created without a device context");

@@ -290,7 +290,7 @@ int tsem_space_set_geometry(tsem_space* sp, const double* xyz) {
   std::fill(sp->cmask.begin(), sp->cmask.end(), (uint8_t)0);
   upload_mask(sp, st);
   d.vpot = nullptr;
-  sp->pmg.reset();
+  sp->drop_pmg();
   sp->loc.valid = false;
   HIPCHK(hipStreamSynchronize(st));
   API_END

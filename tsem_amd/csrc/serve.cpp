@@ -234,9 +234,21 @@ void tsem_server::solve(Job& j, Entry& e) {
     check(tsem_ctx_create(device, &e.ctx), "context");
     if (o.classes) check(tsem_ctx_set_class(e.ctx, fg ? 1 : 2), "solve class");
     check(tsem_space_create(e.ctx, r.M, r.nv, j.xyz.data(), r.nt, j.tet.data(), nullptr, r.qorder, &e.sp), "space");
-    // boundary marks: a geometry lies on a labelled part when all its
-    // vertices do; Dirichlet (1) wins over Neumann (2)
-    Space& S = e.sp->h;
+    const Space& S = e.sp->h;
+    const int64_t N = S.ndof;
+    e.n_pts = std::max<int64_t>(1, std::min<int64_t>((int64_t)r.nt * e.sp->nq3, 2000000));
+    e.n_pts = std::max<int64_t>(e.n_pts, e.sp->nq3);
+    if (hipMalloc(&e.b, N * 8) != hipSuccess || hipMalloc(&e.x, N * 8) != hipSuccess ||
+        hipMalloc(&e.pts, e.n_pts * 24) != hipSuccess || hipMalloc(&e.fq, e.n_pts * 8) != hipSuccess)
+      throw std::bad_alloc();
+  } else {
+    check(tsem_space_set_geometry(e.sp, j.xyz.data()), "geometry");
+  }
+  {
+    // boundary marks of this request (a cached space may have served other
+    // labels or Dirichlet / Neumann parts): a geometry lies on a labelled
+    // part when all its vertices do; Dirichlet (1) wins over Neumann (2)
+    const Space& S = e.sp->h;
     auto mark = [&](int32_t bits) { return (bits & r.dirichlet) ? 1 : (bits & r.neumann) ? 2 : 0; };
     std::vector<int32_t> mv(S.topo.nv), me(S.topo.ne), mf(S.topo.nf);
     auto lab = [&](int v) { return j.labels.empty() ? 0 : j.labels[v]; };
@@ -247,14 +259,6 @@ void tsem_server::solve(Job& j, Entry& e) {
       mf[g] = mark(lab(S.topo.face_vtx[3 * (size_t)g]) & lab(S.topo.face_vtx[3 * (size_t)g + 1]) &
                    lab(S.topo.face_vtx[3 * (size_t)g + 2]));
     check(tsem_set_marks(e.sp, mv.data(), me.data(), mf.data()), "marks");
-    const int64_t N = S.ndof;
-    e.n_pts = std::max<int64_t>(1, std::min<int64_t>((int64_t)r.nt * e.sp->nq3, 2000000));
-    e.n_pts = std::max<int64_t>(e.n_pts, e.sp->nq3);
-    if (hipMalloc(&e.b, N * 8) != hipSuccess || hipMalloc(&e.x, N * 8) != hipSuccess ||
-        hipMalloc(&e.pts, e.n_pts * 24) != hipSuccess || hipMalloc(&e.fq, e.n_pts * 8) != hipSuccess)
-      throw std::bad_alloc();
-  } else {
-    check(tsem_space_set_geometry(e.sp, j.xyz.data()), "geometry");
   }
   j.res.space_s = now_s() - t0;
   tsem_space* sp = e.sp;
