@@ -42,10 +42,20 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 __device__ inline f4 mma(h8 a, h8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 
+// hi and lo must come from one FP32 rounding of the scaled value.  Left to
+// itself the compiler rounds one of the two uses of fp16(v * inv) straight
+// from the exact product (a mixed fp16 FMA) and the other from the FP32
+// product; near fp16 midpoints those differ, and hi + lo is then one fp16
+// ulp off.  The empty asm operand pins the FP32 product in a register.
+__device__ inline float pin(float s) {
+  asm volatile("" : "+v"(s));
+  return s;
+}
+
 __device__ inline void split8(const float* v, float inv, h8& hi, h8& lo) {
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
-    float s = v[t] * inv;
+    float s = pin(v[t] * inv);
     _Float16 h = (_Float16)s;
     hi[t] = h;
     lo[t] = (_Float16)(s - (float)h);
@@ -325,7 +335,7 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
         h4 hi, lo;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const float sv = vv[tp][c][a][j] * inv;
+          const float sv = pin(vv[tp][c][a][j] * inv);  // one FP32 rounding for hi and lo
           const _Float16 h = (_Float16)sv;
           hi[j] = h;
           lo[j] = (_Float16)(sv - (float)h);
