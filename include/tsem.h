@@ -176,6 +176,19 @@ int tsem_zero_rows(tsem_space* sp, int flag, double* x_dev);
  * TSEM::stiff_matrix / mass_matrix (src/TSEMSolver.cpp:1842). */
 int tsem_apply(tsem_space* sp, double stiff, double mass, int constrained, int prec, const void* x, void* y);
 
+/* tsem_apply without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The same kernels and
+ * results as tsem_apply (bit for bit), with the element-local scratch
+ * (gathered element tiles, element results) in one device-resident array of
+ * 64 MiB per device, held until the process ends, instead of the space's
+ * buffers.  Spaces whose scratch exceeds it (about 4 M element-mode entries
+ * in FP64, 8 M in FP32) get TSEM_EINVAL.  Every call on one device shares
+ * that array: calls that overlap on one device (several threads or streams)
+ * race; the array keeps the last call's contents, and every entry a call
+ * reads it first writes. */
+int tsem_apply_noalloc(tsem_space* sp, double stiff, double mass, int constrained, int prec, const void* x, void* y);
+
 /* Element-local coefficients, device: xl[nt][K] (K1 gather). */
 int tsem_gather(tsem_space* sp, const double* x_dev, double* xl_dev);
 /* Transposed gather, device: y = sum_e G_e^T yl_e (K1 scatter, fixed order). */

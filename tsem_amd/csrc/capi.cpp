@@ -163,6 +163,7 @@ int tsem_ctx_create(int device, tsem_ctx** out) {
   HIPCHK(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) throw std::invalid_argument("device index out of range");
   HIPCHK(hipSetDevice(device));
+  tsem::scratch_init(device);
   auto* c = new tsem_ctx();
   c->device = device;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

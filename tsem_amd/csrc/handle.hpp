@@ -27,6 +27,10 @@ namespace tsem {
 // last error message of the calling thread (tsem_last_error)
 extern thread_local std::string g_err;
 
+// looks up the device's copy of tsem_apply_noalloc's scratch array (the
+// device must be current; once per device)
+void scratch_init(int device);
+
 struct DevBuf {
   void* p = nullptr;
   size_t n = 0;

@@ -71,6 +71,7 @@ SIGNATURES = {
     "tsem_background_wait": [_vp, C.c_double],
     "tsem_zero_rows": [_vp, C.c_int, _vp],
     "tsem_apply": [_vp, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _vp],
+    "tsem_apply_noalloc": [_vp, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _vp],
     "tsem_gather": [_vp, _vp, _vp],
     "tsem_scatter": [_vp, _vp, _vp],
     "tsem_diagonal": [_vp, _vp],
