@@ -478,7 +478,7 @@ class Space:
                        converged=(rc == L.TSEM_OK))
 
 
-    def lobpcg(self, m, X=None, seed=7, tol=1e-8, criterion=1, max_iter=1000, precond=4, smooth_steps=2,
+    def lobpcg(self, m, X=None, seed=7, tol=1e-8, criterion=1, max_iter=1000, precond=4, smooth_steps=3,
                stiff=1.0, mass=0.0, potential=0.0, raise_on_noconv=True):
         """solve_LOBPCG: the m lowest eigenpairs of (stiff S + mass M + potential M_V) x
         = lambda M x on the eliminated operators (M_V: set_potential).  X: (m, n_dof) float64 device tensor

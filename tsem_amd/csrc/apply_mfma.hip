@@ -363,7 +363,7 @@ bool mfma_supported(const DevSpace& S) {
 template <typename T, int CT>
 static void run(const DevSpace& S, const OpCoef& op, const T* Dp, T* xt, const T* x, T* yl, hipStream_t st) {
   const int Kl = op.Kl > 0 ? op.Kl : S.K;
-  const bool timed = op.Kl == 0;
+  const bool timed = op.timed();
   const int kind = sizeof(T) == 8 ? TK_GATHER64 : TK_GATHER32;
   int th = timed ? timing_begin(kind, st) : -1;
   const size_t ftc = (size_t)6 * S.ndg2 * (S.ftc_J * (sizeof(T) + 4) + 4) + (size_t)S.K * 4;

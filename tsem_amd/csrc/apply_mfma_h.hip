@@ -609,7 +609,7 @@ bool mfma_h_supported(const DevSpace& S) {
 void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, float* yl, bool slab, hipStream_t st) {
   if (slab && (op.Kl != 0 || !S.sl_meta)) throw std::logic_error("slab results need the full order and slab tables");
   const int Kl = op.Kl > 0 ? op.Kl : S.K;
-  const bool timed = op.Kl == 0;
+  const bool timed = op.timed();
   int th = timed ? timing_begin(TK_GATHER32, st) : -1;
   const size_t ftc = (size_t)6 * S.ndg2 * (S.ftc_J * 8 + 4) + (size_t)S.K * 4;
   hipLaunchKernelGGL(k_gather_tiles_h, dim3((unsigned)S.ntile), dim3(256), (size_t)S.K32 * TILE * 4 + ftc, st, S, Kl, x,

@@ -1086,12 +1086,40 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
     if (g_fused_alpha && g_fused_beta) launch_p_update_beta<T>(n, p, z, sc, part, st);
     else launch_p_update<T>(n, p, z, sc, st);
   };
-  auto& G = (sizeof(T) == 8) ? sp->graph64 : sp->graph32;
+  auto& GS = (sizeof(T) == 8) ? sp->graph64 : sp->graph32;
   double last_chunk = 0.0;  // host seconds of the previous chunk
+  // chunk length: check_every iterations, shortened near the end to the
+  // iterations the observed rate of the last chunk predicts (rounded down to
+  // a power of two, so few graphs are captured), which saves most of the
+  // overshoot past the tolerance
+  const int full = std::max(1, check_every);
+  int nrun = full;
+  double res_prev = res;
+  int run_prev = 0;
   while (res >= tol && it <= max_iter) {
     background_wait(sp->ctx, std::max(last_chunk, 1e-4));
     const auto chunk_t0 = std::chrono::steady_clock::now();
-    int nrun = std::max(1, check_every);
+    nrun = full;
+    if (run_prev > 0 && res < res_prev && res > 0) {
+      const double rate = std::log(res / res_prev) / run_prev;  // < 0
+      const double left = std::log(tol / res) / rate;           // iterations still needed
+      if (left < full) {
+        int q = 1;
+        while (2 * q <= (int)left) q *= 2;
+        nrun = std::max(1, std::min(full, q));
+      }
+    }
+    int gslot = 0;  // slot 0: the full chunk; slot j + 1: a chunk of 2^j iterations
+    if (nrun != full) {
+      int j = 0;
+      while ((1 << j) < nrun) ++j;
+      gslot = j + 1;
+      if (gslot >= tsem_space::GRAPH_SLOTS) {
+        nrun = full;
+        gslot = 0;
+      }
+    }
+    auto& G = GS.slot[gslot];
     std::vector<const void*> key;
     if (g_pcg_graphs && !timing_on() && it + nrun <= max_iter + 1) {
       // the chunk of nrun iterations as one captured graph, replayed while
@@ -1141,6 +1169,8 @@ int pcg_core(tsem_space* sp, const T* b, T* x, double tol, int criterion, double
     HIPCHK(hipMemcpyAsync(&hs, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     last_chunk = std::chrono::duration<double>(std::chrono::steady_clock::now() - chunk_t0).count();
+    res_prev = res;
+    run_prev = nrun;
     if (hook) {
       auto refresh = [&](bool restart) {
         static const double zero = 0.0;
@@ -1196,7 +1226,7 @@ int tsem_pcg(tsem_space* sp, const double* b, double* x, const tsem_pcg_opts* o,
   if (!(o->tol > 0)) throw std::invalid_argument("tol must be positive");
   int n = sp->d.ndof;
   hipStream_t st = sp->st();
-  if (o->precond == 4 && !sp->pmg) sp->pmg.reset(new Pmg(sp->h, sp->d, sp->cmask, st));
+  if (o->precond == 4 && !sp->pmg) sp->pmg.reset(new Pmg(sp, st));
   auto t0 = std::chrono::steady_clock::now();
   tsem_pcg_stats s{};
   int rc = TSEM_OK;

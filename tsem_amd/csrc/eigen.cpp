@@ -214,7 +214,7 @@ int tsem_lobpcg(tsem_space* sp, int m, double* X, double* lambda, const tsem_lob
   const int n = sp->d.ndof;
   if (n < 3 * m) throw std::invalid_argument("block too large for the space");
   auto t0 = std::chrono::steady_clock::now();
-  if (o->precond == 4 && !sp->pmg) sp->pmg.reset(new Pmg(sp->h, sp->d, sp->cmask, st));
+  if (o->precond == 4 && !sp->pmg) sp->pmg.reset(new Pmg(sp, st));
   const size_t nm = (size_t)n * m;
   // trial block [X | P | W] and its images under A and B: X at column 0, P
   // (np columns) at column m, W (nw columns: residuals of the pairs not yet

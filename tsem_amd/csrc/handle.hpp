@@ -113,6 +113,15 @@ struct tsem_space {
       key.clear();
     }
     ~PcgGraph() { reset(); }
+  };
+  // one cached graph per chunk length class (the full check_every chunk and
+  // the shorter power-of-two chunks of the last iterations) and precision
+  static constexpr int GRAPH_SLOTS = 8;
+  struct PcgGraphs {
+    PcgGraph slot[GRAPH_SLOTS];
+    void reset() {
+      for (auto& g : slot) g.reset();
+    }
   } graph64, graph32;
   std::unique_ptr<Pmg> pmg;
   // bumped whenever the operator state a captured PCG chunk depends on

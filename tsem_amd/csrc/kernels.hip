@@ -552,7 +552,7 @@ static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T*
     constexpr int NE = 8;
     size_t lds = (size_t)NE * (2 * S.K + 3 * S.Kd) * sizeof(T);
     int nb = (S.nt + NE - 1) / NE;
-    int th = (op.Kl == 0) ? timing_begin(kind + 1, st) : -1;
+    int th = op.timed() ? timing_begin(kind + 1, st) : -1;
     hipLaunchKernelGGL((k_apply<T, NE>), dim3(nb), dim3(BLK), lds, st, S, op, x, yl);
     timing_end(th, st);
   }
@@ -562,7 +562,7 @@ static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T*
     if (lds > 64 * 1024) throw std::invalid_argument("quadrature too fine for the potential operator (qorder <= 20)");
     hipLaunchKernelGGL(k_apply_potential<T>, dim3(S.nt), dim3(BLK), lds, st, S, op.pot, x, yl, op.constrained);
   }
-  int th = (op.Kl == 0) ? timing_begin(kind + 2, st) : -1;
+  int th = op.timed() ? timing_begin(kind + 2, st) : -1;
   if (slab) slab_scatter<T>(S, yl, y, x, op.constrained, partial, st);
   else scatter_dispatch<T>(S, yl, y, x, op.constrained, partial, st);
   timing_end(th, st);

@@ -123,7 +123,10 @@ struct OpCoef {
   double stiff = 1.0, mass = 0.0;
   double pot = 0.0;      // coefficient of the potential mass operator M_V (DevSpace::vpot)
   int constrained = 0;   // honour the Dirichlet elimination
-  int Kl = 0, Kdl = 0;   // leading modes used (p-multigrid levels); 0 = all
+  int Kl = 0, Kdl = 0;   // leading modes used; 0 = all
+  int level = 0;         // p-multigrid level the apply belongs to (0: the space itself)
+  // applies counted by the per-kernel timing: the space's own full-order operator
+  bool timed() const { return Kl == 0 && level == 0; }
 };
 
 // scalar block of the CG loops (device memory)

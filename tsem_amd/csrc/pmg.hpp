@@ -1,41 +1,69 @@
-// p-multigrid preconditioner on device: levels of polynomial order
-// p, p-1, ..., 1 (TSEMLib init_PMG, src/TSEMSolver.cpp:35-136).  The level-l
-// operator is the principal submatrix of the eliminated operator on the DOFs
-// of degree <= p_l (build_M_exactPartial, src/TSEMSolver.cpp:572-655), applied
-// matrix-free with the leading modes of the same reference factors; transfer
-// is injection / zero extension (src/TSEMSolver.cpp:206-245, 344-382).  The
-// sequential Gauss-Seidel sweeps of ite_PMG (src/TSEMSolver.cpp:1119-1184)
-// become Chebyshev smoothing on the Jacobi-scaled level operator, which is
-// parallel and keeps the V-cycle symmetric.
+// p-multigrid preconditioner on the device (TSEMLib init_PMG / ite_PMG,
+// src/TSEMSolver.cpp:35-136, 1119-1184).
+//
+// Levels of polynomial order p, p / 2, 1 by default (the reference uses p,
+// p-1, ..., 1, n_project = p - 1; TSEM_PMG_ORDERS selects other coarse
+// orders, TSEM_PMG_COARSE the coarsest level's smoothing degree).  The
+// level-l operator is the principal submatrix of the
+// eliminated operator on the DOFs of degree <= p_l (build_M_exactPartial,
+// src/TSEMSolver.cpp:572-655).  The basis is hierarchical and every
+// geometry lists its DOFs by degree, so those DOFs are a prefix of each
+// geometry's DOFs: level l is exactly the order-p_l space on the same mesh,
+// laid out as a space of that order would be, and its operator is that
+// space's operator with the restricted elimination.  Each coarse level is
+// therefore held as a space of its own (compact vectors of N_l DOFs, element
+// kernels at K_l modes, the same matrix-free kernels as the fine level).
+// Transfer is injection and zero extension through an index map per level
+// (build_restrict_identity / build_interpolate_identity,
+// src/TSEMSolver.cpp:206-245, 344-382).  The reference's sequential
+// Gauss-Seidel sweeps become Chebyshev smoothing on the Jacobi-scaled level
+// operator: parallel, and the V-cycle stays symmetric.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <vector>
 
 #include "kernels.hpp"
-#include "space.hpp"
+
+struct tsem_space;
 
 namespace tsem {
 
 class Pmg {
  public:
-  Pmg(const Space& S, const DevSpace& d, const std::vector<uint8_t>& cmask, hipStream_t st);
+  // builds the coarse levels of `fine` (its current elimination and
+  // geometry; rebuilt when either changes)
+  Pmg(tsem_space* fine, hipStream_t st);
   ~Pmg();
-  // z = V(r): one V-cycle from a zero initial guess
+  Pmg(const Pmg&) = delete;
+  Pmg& operator=(const Pmg&) = delete;
+  // z = V(r): one V-cycle from a zero initial guess; `smooth` is the
+  // Chebyshev degree of the pre- and post-smoothing on every level but the
+  // coarsest (which gets coarse_deg)
   template <typename T>
   void vcycle(const T* r, T* z, int smooth, hipStream_t st);
   int levels() const { return (int)order.size(); }
-  std::vector<int> order;       // polynomial order per level
-  std::vector<double> lmax;     // spectral radius estimate of D^-1 A_l
-  std::vector<long> ndof_level;
+  std::vector<int> order;        // polynomial order per level (level 0 = the fine space)
+  std::vector<double> lmax;      // spectral radius estimate of D^-1 A_l
+  std::vector<long> ndof_level;  // N_l
+  int coarse_deg = 20;
 
  private:
-  const DevSpace& d;
-  int n;
-  void* sumdof = nullptr;        // uint8 per DOF
-  std::vector<void*> inv64, inv32;  // masked inverse diagonal per level
-  std::vector<void*> work;       // per level x, b, r, dvec (double and float views share bytes)
-  void* yl = nullptr;            // element-local scratch (double sized)
+  tsem_space* fine;
+  std::vector<tsem_space*> sub;      // levels 1 .. L-1
+  std::vector<int32_t*> map;         // level l >= 1: index in level l-1 of each level-l DOF
+  std::vector<void*> vx, vb, vr, vd;  // per level: iterate, right-hand side, A x, Chebyshev step
+  // the order-1 coarsest level assembled (CSR with the elimination applied:
+  // eliminated rows and columns hold only the diagonal): about 15 entries
+  // per row, where the matrix-free operator would run over every element
+  int csr_level = -1;
+  int32_t* csr_rs = nullptr;
+  int32_t* csr_col = nullptr;
+  double* csr_v64 = nullptr;
+  float* csr_v32 = nullptr;
+  void assemble_level(int l, hipStream_t st);
+  const DevSpace& dev(int l) const;
+  void release();
   template <typename T>
   void apply_level(int l, const T* x, T* y, hipStream_t st);
   template <typename T>
