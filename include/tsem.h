@@ -148,6 +148,23 @@ int tsem_rhs_begin(tsem_space* sp);
 int tsem_rhs_add(tsem_space* sp, int e0, int e1, const double* fq_dev);
 int tsem_rhs_end(tsem_space* sp, double* b_dev);
 
+/* The manufactured Poisson family of the examples and benchmarks:
+ * u = sin(pi x) sin(pi y) sin(pi z) (1 + amp g), g = 1/nterms sum_m
+ * cos(2 pi k_m . x + phi_m), k_m integer in 0..3.  tsem_manufactured
+ * evaluates, at n device points (pts: n x 3), what = 0: u; 1: f = -Lap u;
+ * 2: du/dn with the unit normals (device, n x 3), into out (device, n),
+ * enqueued on `stream` (a hipStream_t; NULL: the null stream) without
+ * waiting.  No counterpart (the reference's example evaluates its own
+ * functions on the host, examples/poisson_mixBC/poisson.cpp:25-27). */
+typedef struct {
+  int nterms;        /* 1..16 */
+  double k[16][3];   /* wave vectors */
+  double phi[16];    /* phases */
+  double amp;
+} tsem_manufactured_def;
+int tsem_manufactured(const tsem_manufactured_def* def, int what, int64_t n, const double* pts, const double* normals,
+                      double* out, void* stream);
+
 /* Neumann load on faces with mark == flag; g at their quadrature points
  * (host, nf_f x q*q, tsem_bc_points order).  b device, updated.
  * TSEM::impose_boundary_condition(rhs, mesh, g, flag) (src/Discretize.cpp:2010-2063). */

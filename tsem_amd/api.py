@@ -250,7 +250,9 @@ class Space:
         b = self.zeros()
         L.check(lib.tsem_rhs_begin(self.h))
         if chunk is None:
-            chunk = max(1, min(self.nt, int(2e8 // max(1, self.nq3 * 3))))
+            # up to 6e8 point coordinates (4.8 GB) per piece: few pieces, each
+            # wide enough for the whole device
+            chunk = max(1, min(self.nt, int(6e8 // max(1, self.nq3 * 3))))
         for e0 in range(0, self.nt, chunk):
             if pause is not None and e0 > 0:
                 pause()
@@ -321,7 +323,9 @@ class Space:
         """calc_l2_error: ||u_h - u|| with u a function of the points (device)."""
         import math
         if chunk is None:
-            chunk = max(1, min(self.nt, int(2e8 // max(1, self.nq3 * 3))))
+            # up to 6e8 point coordinates (4.8 GB) per piece: few pieces, each
+            # wide enough for the whole device
+            chunk = max(1, min(self.nt, int(6e8 // max(1, self.nq3 * 3))))
         tot = 0.0
         s = C.c_double()
         for e0 in range(0, self.nt, chunk):

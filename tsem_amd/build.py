@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtsem_amd.so")
 SOURCES = ["refel.cpp", "space.cpp", "slab.cpp", "capi.cpp", "serve.cpp", "eigen.cpp", "quad_api.cpp", "kernels.hip", "apply_mfma.hip",
-           "apply_mfma_h.hip", "pmg.hip", "block.hip", "quad.hip", "noalloc.hip"]
+           "apply_mfma_h.hip", "pmg.hip", "block.hip", "quad.hip", "noalloc.hip", "source.hip", "bc.hip"]
 EXAMPLES = ["poisson_mixbc", "laplace_eig"]
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -86,24 +87,6 @@ void background_wait(const tsem_ctx* c, double max_seconds) {
 }
 }  // namespace
 
-namespace {
-// standard Jacobi P_n^{(a,b)}(x), a,b > -1
-double jacobi(double a, double b, int n, double x) {
-  if (n == 0) return 1.0;
-  double p0 = 1.0, p1 = (a - b) / 2 + (a + b + 2) / 2 * x;
-  for (int k = 2; k <= n; ++k) {
-    double c = 2 * k + a + b;
-    double A1 = 2 * k * (k + a + b) * (c - 2);
-    double A2 = (c - 1) * (a * a - b * b);
-    double A3 = (c - 1) * c * (c - 2);
-    double A4 = 2 * (k + a - 1) * (k + b - 1) * c;
-    double p2 = ((A2 + A3 * x) * p1 - A4 * p0) / A1;
-    p0 = p1; p1 = p2;
-  }
-  return p1;
-}
-
-}  // namespace
 
 static void ctx_release(tsem_ctx* c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -675,13 +658,28 @@ int tsem_rhs_begin(tsem_space* sp) {
   if (!sp) throw std::invalid_argument("null space");
   const Space& S = sp->h;
   if (!sp->wphi.p) {
-    std::vector<double> wphi((size_t)sp->nq3 * S.K);
-    std::vector<long double> val(S.K);
-    for (int q = 0; q < sp->nq3; ++q) {
-      S.ref.eval(sp->p3[3 * q], sp->p3[3 * q + 1], sp->p3[3 * q + 2], val.data(), nullptr);
-      for (int i = 0; i < S.K; ++i) wphi[(size_t)q * S.K + i] = sp->w3[q] * (double)val[i];
+    // w_q phi_i(x_q) depends on the order and the rule only: built once per
+    // (order, qorder) in the process
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, std::shared_ptr<const std::vector<double>>> cache;
+    std::shared_ptr<const std::vector<double>> tab;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      auto it = cache.find({S.M, sp->qorder});
+      if (it != cache.end()) tab = it->second;
     }
-    upload(sp->wphi, wphi, sp->st());
+    if (!tab) {
+      auto w = std::make_shared<std::vector<double>>((size_t)sp->nq3 * S.K);
+      std::vector<long double> val(S.K);
+      for (int q = 0; q < sp->nq3; ++q) {
+        S.ref.eval(sp->p3[3 * q], sp->p3[3 * q + 1], sp->p3[3 * q + 2], val.data(), nullptr);
+        for (int i = 0; i < S.K; ++i) (*w)[(size_t)q * S.K + i] = sp->w3[q] * (double)val[i];
+      }
+      std::lock_guard<std::mutex> lk(mu);
+      tab = cache.emplace(std::make_pair(S.M, sp->qorder), w).first->second;
+    }
+    upload(sp->wphi, *tab, sp->st());
+    HIPCHK(hipStreamSynchronize(sp->st()));  // the table is read from host memory
   }
   size_t nbl = (size_t)sp->d.ntile * TILE * S.K;
   sp->bl.alloc(nbl * sizeof(double));
@@ -696,7 +694,10 @@ int tsem_rhs_add(tsem_space* sp, int e0, int e1, const double* fq) {
   if (!sp || !fq) throw std::invalid_argument("null argument");
   if (!sp->rhs_open) { g_err = "tsem_rhs_begin not called"; return TSEM_ESTATE; }
   if (e0 < 0 || e1 > sp->h.topo.nt || e0 > e1) throw std::invalid_argument("bad element range");
-  launch_rhs_local(sp->d, e0, e1, sp->nq3, sp->wphi.as<double>(), fq, sp->bl.as<double>(), sp->st());
+  const int ns = rhs_splits(sp->d, e1 - e0, sp->nq3);
+  if (ns > 1) sp->rhs_part.alloc((size_t)ns * (e1 - e0) * sp->h.K * sizeof(double));
+  launch_rhs_local(sp->d, e0, e1, sp->nq3, sp->wphi.as<double>(), fq, sp->bl.as<double>(),
+                   ns > 1 ? sp->rhs_part.as<double>() : nullptr, sp->st());
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(sp->st()));
   API_END
@@ -714,68 +715,7 @@ int tsem_rhs_end(tsem_space* sp, double* b) {
 }
 
 // ---------------------------------------------------------------- BCs ------
-int tsem_neumann(tsem_space* sp, const double* gq, int flag, double* b_dev) {
-  API_BEGIN
-  if (!sp || !gq || !b_dev) throw std::invalid_argument("null argument");
-  Space& S = sp->h;
-  if (S.mark[2].empty()) { g_err = "boundary marks not set"; return TSEM_ESTATE; }
-  int M = S.M, q2 = sp->nq2, nde = S.ndg[1], ndf = S.ndg[2];
-  // face-local bases at the 2-D points: face modes (TSEM basis_value[1]),
-  // edge traces (-2 x y J^{1,1}_{l-2}(eta) ...) as src/Discretize.cpp:2030-2041
-  std::vector<double> bf((size_t)q2 * ndf), ea((size_t)q2 * nde), eb((size_t)q2 * nde);
-  std::vector<long double> val(S.K);
-  for (int k = 0; k < q2; ++k) {
-    double x = sp->p2[3 * k], y = sp->p2[3 * k + 1];
-    S.ref.eval(x, y, 0.0L, val.data(), nullptr);
-    for (int l1 = 2; l1 <= M; ++l1)
-      for (int l2 = 1; l1 + l2 <= M; ++l2)
-        bf[(size_t)k * ndf + (l1 + l2 - 3) * (l1 + l2 - 2) / 2 + l2 - 1] = (double)val[S.ref.idx.number(l1, l2, 0)];
-    double xi = 2 * x / (1 - y) - 1, eta = 2 * y - 1;
-    for (int l = 2; l <= M; ++l) {
-      ea[(size_t)k * nde + l - 2] = std::pow(1 - y, l - 2) * jacobi(1, 1, l - 2, xi);
-      eb[(size_t)k * nde + l - 2] = jacobi(1, 1, l - 2, eta);
-    }
-  }
-  std::vector<double> b(S.ndof);
-  HIPCHK(hipMemcpyAsync(b.data(), b_dev, S.ndof * 8, hipMemcpyDeviceToHost, sp->st()));
-  HIPCHK(hipStreamSynchronize(sp->st()));
-  auto faces = marked(S, 2, flag);
-  for (size_t jf = 0; jf < faces.size(); ++jf) {
-    int f = faces[jf];
-    const double* p0 = &S.xyz[3 * (size_t)S.face_node[f][0]];
-    const double* p1 = &S.xyz[3 * (size_t)S.face_node[f][1]];
-    const double* p2 = &S.xyz[3 * (size_t)S.face_node[f][2]];
-    double u[3], v[3];
-    for (int c = 0; c < 3; ++c) { u[c] = p1[c] - p0[c]; v[c] = p2[c] - p0[c]; }
-    double cx = u[1] * v[2] - u[2] * v[1], cy = u[0] * v[2] - u[2] * v[0], cz = u[0] * v[1] - u[1] * v[0];
-    double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
-    double cv[3] = {0, 0, 0};
-    std::vector<double> ce(3 * (size_t)nde, 0.0), cf(ndf, 0.0);
-    for (int k = 0; k < q2; ++k) {
-      const double* bq = &sp->p2[3 * (size_t)k];
-      double wg = sp->w2[k] * gq[jf * q2 + k] * area;
-      for (int j = 0; j < 3; ++j) cv[j] += wg * bq[(j + 2) % 3];
-      double x = bq[0], y = bq[1], r = bq[2];
-      for (int dd = 0; dd < nde; ++dd) {
-        ce[dd] += wg * -2 * x * y * eb[(size_t)k * nde + dd];
-        ce[nde + dd] += wg * -2 * y * r * eb[(size_t)k * nde + dd];
-        ce[2 * nde + dd] += wg * -2 * x * r * ea[(size_t)k * nde + dd];
-      }
-      for (int dd = 0; dd < ndf; ++dd) cf[dd] += wg * bf[(size_t)k * ndf + dd];
-    }
-    for (int j = 0; j < 3; ++j) b[S.geo_dof(0, S.face_node[f][j])] += cv[j];
-    for (int ie = 0; ie < 3; ++ie) {
-      bool rev = (S.face_rev[f] >> ie) & 1;
-      int64_t st0 = S.geo_dof(1, S.face_edges[f][ie]);
-      for (int dd = 0; dd < nde; ++dd) b[st0 + dd] += (rev && (dd % 2 == 1)) ? -ce[ie * nde + dd] : ce[ie * nde + dd];
-    }
-    int64_t fs = S.geo_dof(2, f);
-    for (int dd = 0; dd < ndf; ++dd) b[fs + dd] += cf[dd];
-  }
-  HIPCHK(hipMemcpyAsync(b_dev, b.data(), S.ndof * 8, hipMemcpyHostToDevice, sp->st()));
-  HIPCHK(hipStreamSynchronize(sp->st()));
-  API_END
-}
+// tsem_neumann and tsem_dirichlet are in bc.hip
 
 static void apply_dev(tsem_space* sp, double stiff, double mass, int constrained, const double* x, double* y,
                       double* partial) {
@@ -783,118 +723,6 @@ static void apply_dev(tsem_space* sp, double stiff, double mass, int constrained
   op.stiff = stiff; op.mass = mass; op.constrained = constrained;
   launch_apply<double>(sp->d, op, x, y, sp->yl64.as<double>(), partial, sp->st());
   HIPCHK(hipGetLastError());
-}
-
-int tsem_dirichlet(tsem_space* sp, const double* uv, const double* ue, const double* uf, int flag, double* b_dev,
-                   double* bval) {
-  API_BEGIN
-  if (!sp || !uv || !ue || !uf || !b_dev) throw std::invalid_argument("null argument");
-  Space& S = sp->h;
-  if (S.mark[0].empty()) { g_err = "boundary marks not set"; return TSEM_ESTATE; }
-  int M = S.M, q1 = sp->nq1, q2 = sp->nq2, nde = S.ndg[1], ndf = S.ndg[2];
-  hipStream_t st = sp->st();
-  std::vector<double> b(S.ndof);
-  HIPCHK(hipMemcpyAsync(b.data(), b_dev, S.ndof * 8, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  const std::vector<double>& dg = sp->diag;
-  // interpolation bases: J^{1,1}_{l-2} on edges; on faces
-  // (1-eta)^{l1-2} J^{1,1}_{l1-2}(xi) J^{2l1-1,1}_{l2-1}(eta) and the edge
-  // traces used to remove the edge part (src/BuildSpace.cpp:95-147)
-  std::vector<double> ie1((size_t)q1 * nde), if2((size_t)q2 * std::max(ndf, 1)), ea((size_t)q2 * nde), eb((size_t)q2 * nde);
-  for (int k = 0; k < q1; ++k)
-    for (int l = 2; l <= M; ++l) ie1[(size_t)k * nde + l - 2] = jacobi(1, 1, l - 2, 2 * sp->t1[k] - 1);
-  for (int k = 0; k < q2; ++k) {
-    double x = sp->p2[3 * k], y = sp->p2[3 * k + 1];
-    double xi = 2 * x / (1 - y) - 1, eta = 2 * y - 1;
-    for (int l1 = 2; l1 <= M; ++l1)
-      for (int l2 = 1; l1 + l2 <= M; ++l2)
-        if2[(size_t)k * ndf + (l1 + l2 - 3) * (l1 + l2 - 2) / 2 + l2 - 1] =
-            std::pow(1 - eta, l1 - 2) * jacobi(1, 1, l1 - 2, xi) * jacobi(2 * l1 - 1, 1, l2 - 1, eta);
-    for (int l = 2; l <= M; ++l) {
-      ea[(size_t)k * nde + l - 2] = std::pow(1 - y, l - 2) * jacobi(1, 1, l - 2, xi);
-      eb[(size_t)k * nde + l - 2] = jacobi(1, 1, l - 2, eta);
-    }
-  }
-  std::vector<double> vb(S.ndof, 0.0);
-  std::vector<uint8_t> newmask(S.ndof, 0);
-  if (bval) for (int64_t i = 0; i < S.ndof; ++i) bval[i] = NAN;
-  for (int dim = 0; dim <= 2; ++dim) {
-    auto geos = marked(S, dim, flag);
-    for (size_t jg = 0; jg < geos.size(); ++jg) {
-      int g = geos[jg];
-      int64_t row0 = S.geo_dof(dim, g);
-      std::vector<double> val(S.ndg[dim], 0.0);
-      if (dim == 0) val[0] = uv[jg];
-      if (dim == 1) {
-        int64_t ls = S.geo_dof(0, S.edge_node[g][0]), le = S.geo_dof(0, S.edge_node[g][1]);
-        double cs = b[ls] / dg[ls], ce = b[le] / dg[le];
-        for (int l = 2; l <= M; ++l) {
-          double c = 0;
-          for (int k = 0; k < q1; ++k)
-            c += sp->w1[k] * (ue[jg * q1 + k] - cs * (1 - sp->t1[k]) - ce * sp->t1[k]) * ie1[(size_t)k * nde + l - 2];
-          val[l - 2] = c * -l * (2 * l - 1) / (2 * (l - 1));
-        }
-      }
-      if (dim == 2) {
-        double cvt[3];
-        for (int j = 0; j < 3; ++j) { int64_t p = S.geo_dof(0, S.face_node[g][j]); cvt[j] = b[p] / dg[p]; }
-        std::vector<double> ved(3 * (size_t)nde);
-        for (int ie = 0; ie < 3; ++ie) {
-          int64_t es = S.geo_dof(1, S.face_edges[g][ie]);
-          bool rev = (S.face_rev[g] >> ie) & 1;
-          for (int l = 2; l <= M; ++l) {
-            double v = b[es + l - 2] / dg[es + l - 2];
-            ved[ie * nde + l - 2] = (rev && l % 2 == 1) ? -v : v;
-          }
-        }
-        for (int l1 = 2; l1 <= M; ++l1)
-          for (int l2 = 1; l1 + l2 <= M; ++l2) {
-            int id = (l1 + l2 - 3) * (l1 + l2 - 2) / 2 + l2 - 1;
-            double c = 0;
-            for (int k = 0; k < q2; ++k) {
-              const double* bq = &sp->p2[3 * (size_t)k];
-              double x = bq[0], y = bq[1], r = bq[2];
-              double cp = uf[jg * q2 + k];
-              for (int j = 0; j < 3; ++j) cp -= cvt[j] * bq[(j + 2) % 3];
-              for (int l = 2; l <= M; ++l)
-                cp += 2 * (x * y * ved[l - 2] * eb[(size_t)k * nde + l - 2] + y * r * ved[nde + l - 2] * eb[(size_t)k * nde + l - 2] +
-                           x * r * ved[2 * nde + l - 2] * ea[(size_t)k * nde + l - 2]);
-              c += sp->w2[k] * cp * if2[(size_t)k * ndf + id];
-            }
-            val[id] = c * -l1 * (2 * l1 - 1) * (2 * l1 + 2 * l2 - 1) / (std::pow(2, l1) * (l1 - 1));
-          }
-      }
-      for (int i = 0; i < S.ndg[dim]; ++i) {
-        int64_t row = row0 + i;
-        b[row] = dg[row] * val[i];
-        vb[row] = val[i];
-        newmask[row] = 1;
-        if (bval) bval[row] = val[i];
-      }
-    }
-  }
-  // free rows: b -= A ext(val), with the elimination imposed so far on the
-  // other DOFs (the DOFs imposed here are unmasked for this product, so data
-  // imposed again on DOFs eliminated before still reach the free rows)
-  std::vector<uint8_t> prev(sp->cmask);
-  for (int64_t i = 0; i < S.ndof; ++i)
-    if (newmask[i]) sp->cmask[i] = 0;
-  upload_mask(sp, st);
-  DevBuf tmpv, tmpw;
-  upload(tmpv, vb, st);
-  tmpw.alloc(S.ndof * 8);
-  apply_dev(sp, 1.0, 0.0, 1, tmpv.as<double>(), tmpw.as<double>(), nullptr);
-  std::vector<double> w(S.ndof);
-  HIPCHK(hipMemcpyAsync(w.data(), tmpw.p, S.ndof * 8, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  for (int64_t i = 0; i < S.ndof; ++i)
-    if (!newmask[i] && !sp->cmask[i]) b[i] -= w[i];
-  for (int64_t i = 0; i < S.ndof; ++i) sp->cmask[i] = prev[i] | newmask[i];
-  upload_mask(sp, st);
-  HIPCHK(hipMemcpyAsync(b_dev, b.data(), S.ndof * 8, hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));
-  sp->drop_pmg();
-  API_END
 }
 
 int tsem_zero_bc(tsem_space* sp, int flag) {

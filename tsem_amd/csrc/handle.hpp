@@ -86,8 +86,12 @@ struct tsem_space {
   DevBuf desc, slot, ft_ptr, ft_key, ft_w, ghat, ghat32, sixv, sixv32, dT, dN, dT32, dN32, mT, mN, mT32, mN32;
   DevBuf sc_ptr, sc_code, sc_w, sc_w32, diag_d, invdiag, invdiag32, cmask_d, xe, pts3, wphi, Dp64, Dp32;
   DevBuf yl64, yl32, bl, partial, scal, xt64, xt32, gmask;
+  DevBuf rhs_part;  // point-range partial sums of the load-vector GEMM
   // quadrature-point work: basis at the points, weights, potential, scratch
   DevBuf qphi, qphiT, qdphi, qw, vpot, qxl, qpart, qout, grad2;
+  // boundary-condition quadrature and modal bases (bc.hip), offsets per table
+  DevBuf bc_tab, bc_offd;
+  std::vector<long> bc_off;
   DevBuf Dh1hi, Dh1lo, Dh2hi, Dh2lo, xhhi, xhlo, xhs;  // fp16-split FP32 operator
   tsem::Slab slab;                                      // entity-slab layout (host tables)
   // point location: element bounding boxes bucketed on a uniform grid

@@ -903,24 +903,7 @@ void launch_residual(const DevSpace& S, int n, const double* b, const double* Ax
 }
 
 // -------------------------------------------------------------------- RHS ----
-// bl[e][i] = vol_e * sum_q (w_q phi_q[i]) f[e][q]
-__global__ void __launch_bounds__(BLK) k_rhs_local(DevSpace S, int e0, int e1, int nq, const double* __restrict__ wphi,
-                                                   const double* __restrict__ fq, double* __restrict__ bl) {
-  long s = (long)blockIdx.x * BLK + threadIdx.x;
-  long n = (long)(e1 - e0) * S.K;
-  if (s >= n) return;
-  int le = (int)(s / S.K), i = (int)(s % S.K);
-  int e = e0 + le;
-  const double* f = fq + (size_t)le * nq;
-  double acc = 0;
-  for (int q = 0; q < nq; ++q) acc += wphi[(size_t)q * S.K + i] * f[q];
-  bl[(size_t)e * S.K + i] = acc * (S.sixv[e] / 6.0);
-}
-void launch_rhs_local(const DevSpace& S, int e0, int e1, int nq, const double* wphi, const double* fq, double* bl,
-                      hipStream_t st) {
-  long n = (long)(e1 - e0) * S.K;
-  hipLaunchKernelGGL(k_rhs_local, dim3((unsigned)((n + BLK - 1) / BLK)), dim3(BLK), 0, st, S, e0, e1, nq, wphi, fq, bl);
-}
+// (the load-vector kernels are in source.hip)
 
 // physical coordinates of reference points on elements [e0, e1)
 __global__ void k_qp_coords(const double* __restrict__ xe, int e0, int e1, int nq, const double* __restrict__ pts,

@@ -211,8 +211,11 @@ void launch_gap(int n, const float* r32, const double* r64, double s, double* pa
 void launch_residual(const DevSpace& S, int n, const double* b, const double* Ax, double* r, double* partial, hipStream_t st);
 
 // RHS: bl[e][i] = 6V/6 * sum_q w_q f[e][q] phi_q[i] for elements [e0, e1)
+// part: scratch for the point-range partial sums (rhs_splits(S, e1 - e0, nq)
+// x (e1 - e0) x K doubles), or null for one range
+int rhs_splits(const DevSpace& S, int ne, int nq);
 void launch_rhs_local(const DevSpace& S, int e0, int e1, int nq, const double* wphi /*nq x K, w folded*/,
-                      const double* fq, double* bl, hipStream_t st);
+                      const double* fq, double* bl, double* part, hipStream_t st);
 void launch_qp_coords(const DevSpace& S, const double* xyz_elem /*nt x 12*/, int e0, int e1, int nq,
                       const double* pts /*nq x 3*/, double* out, hipStream_t st);
 

@@ -39,6 +39,10 @@ class LobpcgStats(C.Structure):
                 ("max_dlambda", C.c_double), ("seconds", C.c_double)]
 
 
+class ManufacturedDef(C.Structure):
+    _fields_ = [("nterms", C.c_int), ("k", C.c_double * 48), ("phi", C.c_double * 16), ("amp", C.c_double)]
+
+
 # exported entry points: name -> argument types (all return int status unless noted)
 _vp, _ip, _dp, _lp = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_int64)
 SIGNATURES = {
@@ -72,6 +76,7 @@ SIGNATURES = {
     "tsem_zero_rows": [_vp, C.c_int, _vp],
     "tsem_apply": [_vp, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _vp],
     "tsem_apply_noalloc": [_vp, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _vp],
+    "tsem_manufactured": [C.POINTER(ManufacturedDef), C.c_int, C.c_int64, _vp, _vp, _vp, _vp],
     "tsem_gather": [_vp, _vp, _vp],
     "tsem_scatter": [_vp, _vp, _vp],
     "tsem_diagonal": [_vp, _vp],

@@ -5,7 +5,8 @@ g(x) = 1/8 sum_{m<8} cos(2 pi k_m . x + phi_m),  k_m in {1,2,3}^3, phi_m in [0, 
 
 drawn from ``numpy.random.default_rng(seed)`` (k first, then phi).  f = -Lap u
 and the Neumann data du/dn are exact.  Every function accepts numpy or torch
-arrays of points with a trailing dimension of 3.
+arrays of points with a trailing dimension of 3; float64 device tensors are
+evaluated by the library in one pass (tsem_manufactured).
 """
 import math
 
@@ -49,7 +50,42 @@ class Manufactured:
             LG = LG - 4 * pi * pi * float(k @ k) * c / nt
         return S, gS, G, gG, LG
 
+    def _device(self, p, what, normal=None):
+        """u, f or du/dn at device points by the library's one-pass
+        evaluator (tsem_manufactured), on torch's current stream; None when p
+        is not a float64 device tensor."""
+        import torch
+        if not (isinstance(p, torch.Tensor) and p.is_cuda and p.dtype == torch.float64):
+            return None
+        import ctypes as C
+        from . import lib
+        d = getattr(self, "_def", None)
+        if d is None:
+            d = lib.ManufacturedDef()
+            d.nterms = len(self.phi)
+            for m in range(len(self.phi)):
+                for j in range(3):
+                    d.k[3 * m + j] = float(self.k[m][j])
+                d.phi[m] = float(self.phi[m])
+            d.amp = float(self.amp)
+            self._def = d
+        pts = p.reshape(-1, 3).contiguous()
+        out = torch.empty(pts.shape[0], dtype=torch.float64, device=p.device)
+        if pts.shape[0] == 0:
+            return out.reshape(p.shape[:-1])
+        nrm = None
+        if normal is not None:
+            nrm = torch.stack([torch.broadcast_to(c, p.shape[:-1]) for c in normal], -1).reshape(-1, 3).contiguous()
+        st = torch.cuda.current_stream(p.device).cuda_stream
+        lib.check(lib.load().tsem_manufactured(C.byref(d), what, pts.shape[0], C.c_void_p(pts.data_ptr()),
+                                               C.c_void_p(nrm.data_ptr()) if nrm is not None else None,
+                                               C.c_void_p(out.data_ptr()), C.c_void_p(st)))
+        return out.reshape(p.shape[:-1])
+
     def u(self, p):
+        v = self._device(p, 0)
+        if v is not None:
+            return v
         S, _, G, _, _ = self._parts(p)
         return S * (1 + self.amp * G)
 
@@ -58,12 +94,18 @@ class Manufactured:
         return [gS[d] * (1 + self.amp * G) + self.amp * S * gG[d] for d in range(3)]
 
     def f(self, p):
+        v = self._device(p, 1)
+        if v is not None:
+            return v
         S, gS, G, gG, LG = self._parts(p)
         lap = -3 * math.pi ** 2 * S * (1 + self.amp * G) + 2 * self.amp * sum(gS[d] * gG[d] for d in range(3)) \
             + self.amp * S * LG
         return -lap
 
     def dudn(self, p, normal):
+        v = self._device(p, 2, normal)
+        if v is not None:
+            return v
         g = self.grad(p)
         return g[0] * normal[0] + g[1] * normal[1] + g[2] * normal[2]
 
