@@ -318,11 +318,9 @@ def main():
     slab = sp.slab_state()[0]
     nch = (N + 255) // 256
     algo = {
-        # k_gather_tiles_h: x once, element descriptors (88 B) + geometry mask; the fp16 hi and lo halves
-        # of the scaled tile (2 x 2 B per padded coefficient) and one float scale per element written
-        "k_gather_tiles_h": ("hbm", 4.0 * N + 92.0 * E + 4.0 * E * K32 + 4.0 * E),
-        # k_elem_gemm_h<CT>: two GEMMs through the derivative factors, 12 Kd K FP32-equivalent flops per
-        # element (the fp16 MFMA work is three times that: hi*hi, hi*lo and lo*hi products)
+        # k_elem_gemm_h<CT>: the gather (x through the element descriptors, into LDS), two GEMMs
+        # through the derivative factors and the slab epilogue in one kernel; 12 Kd K FP32-equivalent
+        # flops per element (the fp16 MFMA work is three times that: hi*hi, hi*lo and lo*hi products)
         "k_elem_gemm_h<%d>" % ct: ("mfma", 12.0 * Kd * K * E),
     }
     if slab:
@@ -335,8 +333,9 @@ def main():
         algo["k_scatter_chunks<float>"] = ("hbm", 4.0 * E * K + 4.0 * nnz_sc + 4.0 * (N + 1) + 4.0 * N + 4.0 * N
                                            + 1.0 * N)
     kernels = {}
-    for i, name in enumerate(algo):
-        tot, cnt = kt[6 + 2 * i], kt[7 + 2 * i]
+    slots = {"k_elem_gemm_h<%d>" % ct: 8, "k_scatter_slab<float>": 10, "k_scatter_chunks<float>": 10}
+    for name in algo:
+        tot, cnt = kt[slots[name]], kt[slots[name] + 1]
         bound, work = algo[name]
         avg = tot / cnt if cnt else None
         rate = work / (avg * 1e-3) / (1e12 if bound == "mfma" else 1e9) if avg else None
@@ -368,7 +367,8 @@ def main():
         return 2.0 * N * sz + 6.0 * E * sz + 4.0 * E * K + 12.0 * fx + 11.0 * N * sz
 
     it_bytes = b_iter(4.0)
-    inter_bytes = 2 * (4.0 * E * K32 + 4.0 * E) + (2 * 4.0 * E * K if slab else 2 * 4.0 * E * K)
+    # the element results (entity slab or element-major), written and read back
+    inter_bytes = 2 * 4.0 * E * K
     # the Jacobi mixed solve's iterations are exactly these bytes each
     jac_t, jac_st = mixed_runs.get(1, (None, None))
     n_it = jac_st[-1]["iterations"] if jac_st else None

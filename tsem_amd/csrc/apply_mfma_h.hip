@@ -8,11 +8,10 @@
 // the rounding of the parts leave a relative error of about 2^-22 per
 // product (FP32 itself: 2^-24), at 16x the FP32 MFMA rate per instruction.
 //
-// k_gather_tiles_h: element coefficients of a 32-element tile, scaled by the
-//   element's max |x|, split into hi/lo fp16 groups of 8 consecutive modes:
-//   xh[tile][k / 8][element][8] (the B-operand layout of the MFMA), scale
-//   xs[e] = max |x_e|.
-// k_elem_gemm_h: per block of NE = 16 CT elements,
+// k_elem_gemm_h: per block of NE = 16 CT elements, the element coefficients
+//   gathered from x (element descriptors, face table rows in LDS), scaled by
+//   the element's max |x| and split into hi/lo fp16 in LDS in the MFMA B
+//   layout [k / 8][element][8];
 //   U = D X (GEMM 1), V = Ghat U per element (FP32, kept in registers),
 //   exact per-element max |V| (LDS atomics), V / max split to fp16 hi/lo in
 //   LDS (element-major, the B-operand layout), Y = D^T V (GEMM 2), Y rescaled
@@ -68,151 +67,11 @@ __device__ inline void st8(uint16_t* p, h8 v) { *reinterpret_cast<h8*>(p) = v; }
 __device__ inline void st4(uint16_t* p, h4 v) { *reinterpret_cast<h4*>(p) = v; }
 }  // namespace
 
-// one block per 32-element tile
-__global__ void __launch_bounds__(256) k_gather_tiles_h(DevSpace S, int Kl, const float* __restrict__ x,
-                                                        uint16_t* __restrict__ xhi, uint16_t* __restrict__ xlo,
-                                                        float* __restrict__ xs, int constrained) {
-  __shared__ ElemDesc sd[TILE];
-  __shared__ int sm[TILE];
-  __shared__ float smax[TILE];
-  extern __shared__ float vf[];  // [K32][TILE], then the face table rows: w [R J], n [R], key [R J], slot codes [K]
-  const int tile = xcd_block(blockIdx.x, gridDim.x);
-  const int R = 6 * S.ndg2, J = S.ftc_J;
-  float* cw = vf + (size_t)S.K32 * TILE;
-  int* cn = reinterpret_cast<int*>(cw + (size_t)R * J);
-  int* ck = cn + R;
-  int* sslot = ck + R * J;  // the slot codes of the local modes: no global load per entry
-  // faces in a diagonal variant (bit v of sl_diag: each local face mode is
-  // +- the global face DOF of the same index, sign bit d of sl_sgn[v]) skip
-  // the table rows: one load, the same value as the one-term row gives
-  __shared__ uint32_t ssgn[6];
-  const int dg = S.sl_diag;
-  const int e0 = tile * TILE;
-  const int nvalid = min(TILE, S.nt - e0);
-  const int tid = threadIdx.x;
-  // the set-up loads (element descriptors, masks, the tables' first
-  // entries) are all issued before any of them is waited for, then written
-  // to LDS; table entries past the first TQ per thread (high orders) follow
-  // in plain loops
-  constexpr int W = sizeof(ElemDesc) / 4;
-  constexpr int DQ = (TILE * W + 255) / 256;
-  constexpr int TQ = 2;
-  const int32_t* src = reinterpret_cast<const int32_t*>(S.desc + e0);
-  int32_t* dst = reinterpret_cast<int32_t*>(sd);
-  int32_t dv[DQ];
-#pragma unroll
-  for (int q = 0; q < DQ; ++q) {
-    const int s = tid + 256 * q;
-    dv[q] = (s < TILE * W && s / W < nvalid) ? src[s] : 0;
-  }
-  const int gmv = (constrained && tid < nvalid) ? S.gmask[e0 + tid] : 0;
-  float cwv[TQ];
-  int ckv[TQ];
-#pragma unroll
-  for (int q = 0; q < TQ; ++q) {
-    const int s = tid + 256 * q;
-    cwv[q] = s < R * J ? S.ftc_w32[s] : 0.0f;
-    ckv[q] = s < R * J ? S.ftc_key[s] : 0;
-  }
-  const int cnv = tid < R ? S.ftc_n[tid] : 0;
-  const int slv = tid < S.K ? S.slot[tid] : 0;
-  const uint32_t sgv = tid < 6 ? S.sl_sgn[tid] : 0u;
-#pragma unroll
-  for (int q = 0; q < DQ; ++q) {
-    const int s = tid + 256 * q;
-    if (s < TILE * W) dst[s] = dv[q];
-  }
-  if (tid < TILE) sm[tid] = gmv;
-#pragma unroll
-  for (int q = 0; q < TQ; ++q) {
-    const int s = tid + 256 * q;
-    if (s < R * J) { cw[s] = cwv[q]; ck[s] = ckv[q]; }
-  }
-  for (int s = tid + 256 * TQ; s < R * J; s += 256) { cw[s] = S.ftc_w32[s]; ck[s] = S.ftc_key[s]; }
-  if (tid < R) cn[tid] = cnv;
-  for (int s = tid + 256; s < R; s += 256) cn[s] = S.ftc_n[s];
-  if (tid < S.K) sslot[tid] = slv;
-  for (int s = tid + 256; s < S.K; s += 256) sslot[s] = S.slot[s];
-  if (tid < 6) ssgn[tid] = sgv;
-  __syncthreads();
-  const int K32 = S.K32;
-  const int K32l = (Kl + 31) & ~31;
-  for (int r = threadIdx.x; r < K32l * TILE; r += 256) {
-    int k = r / TILE, le = r % TILE;
-    float v = 0.0f;
-    if (le < nvalid && k < Kl) {
-      const ElemDesc& D = sd[le];
-      const int msk = sm[le];
-      int code = sslot[k];
-      int kind = code & 3, geo = (code >> 2) & 7, sub = code >> 5;
-      if (kind == 0) {
-        v = ((msk >> geo) & 1) ? 0.0f : x[D.v[geo]];
-      } else if (kind == 1) {
-        int ed = D.e[geo];
-        v = ((msk >> (4 + geo)) & 1) ? 0.0f : x[(ed & 0x7fffffff) + sub];
-        if (ed < 0 && (sub & 1)) v = -v;
-      } else if (kind == 3) {
-        v = ((msk >> 14) & 1) ? 0.0f : x[D.t + sub];
-      } else {
-        int fo = D.fo[geo];
-        const bool fmask = (msk >> (10 + geo)) & 1;
-        if ((dg >> (fo & 7)) & 1) {
-          v = fmask ? 0.0f : x[D.f[geo] + sub];
-          if (sub < 32 && ((ssgn[fo & 7] >> sub) & 1)) v = -v;
-        } else {
-          const int row = (fo & 7) * S.ndg2 + sub;
-          float acc = 0.0f;
-          const int nterm = cn[row];
-          for (int j = 0; j < nterm; ++j) {
-            int key = ck[row * J + j];
-            int ie = key & 3, idx = key >> 2;
-            float w = cw[row * J + j];
-            float xv;
-            if (ie < 3) {
-              int le2 = (fo >> (3 + 3 * ie)) & 7;
-              if (((fo >> (12 + ie)) & 1) && (idx & 1)) w = -w;
-              xv = ((msk >> (4 + le2)) & 1) ? 0.0f : x[(D.e[le2] & 0x7fffffff) + idx];
-            } else {
-              xv = fmask ? 0.0f : x[D.f[geo] + idx];
-            }
-            acc += xv * w;
-          }
-          v = acc;
-        }
-      }
-    }
-    vf[k * TILE + le] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < TILE) {
-    float m = 0.0f;
-    for (int k = 0; k < K32l; ++k) m = fmaxf(m, fabsf(vf[k * TILE + threadIdx.x]));
-    smax[threadIdx.x] = m;
-    if (e0 + (int)threadIdx.x < S.nt) xs[e0 + threadIdx.x] = m;
-  }
-  __syncthreads();
-  // groups of 8 modes of one element: xh[tile][kb][le][8]
-  const int KB = K32 / 8, KBl = K32l / 8;
-  for (int g = threadIdx.x; g < KBl * TILE; g += 256) {
-    int kb = g / TILE, le = g % TILE;
-    float v[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) v[t] = vf[(8 * kb + t) * TILE + le];
-    float m = smax[le];
-    h8 hi, lo;
-    split8(v, m > 0.0f ? 1.0f / m : 0.0f, hi, lo);
-    size_t o = (((size_t)tile * KB + kb) * TILE + le) * 8;
-    st8(xhi + o, hi);
-    st8(xlo + o, lo);
-  }
-}
-
 // at least 3 waves per SIMD: the phases are latency bound, occupancy pays
 // more than registers (measured: 162 -> 128 us per launch on the 32^3, p = 6
 // mesh, no spills)
 template <int CT>
-__global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, const uint16_t* __restrict__ xhi,
-                                                    const uint16_t* __restrict__ xlo, const float* __restrict__ xs,
+__global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, const float* __restrict__ x,
                                                     float* __restrict__ yl, int slab) {
   constexpr int NE = 16 * CT;
   constexpr int VS = NE + 1;  // odd row stride of the FP32 LDS tiles: no bank conflicts
@@ -247,6 +106,134 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
       eopre[q] = (slab && s2 < nv0 * 17) ? S.sl_eoff[(size_t)e0 * 17 + s2] : 0;
     }
   }
+  // ---- phase 0: the block's element coefficients, gathered from x through
+  // the element descriptors and the face table, scaled by each element's max
+  // |x| and split to fp16 hi/lo straight into LDS in the MFMA B layout
+  // [k / 8][element][8].  The region is the V tile's, dead until phase B.
+  const int R = 6 * S.ndg2, J = S.ftc_J;
+  uint16_t* xhh = reinterpret_cast<uint16_t*>(smem_raw);
+  uint16_t* xhl = xhh + (size_t)KB * NE * 8;
+  ElemDesc* sd = reinterpret_cast<ElemDesc*>(xhl + (size_t)KB * NE * 8);
+  int* sgm = reinterpret_cast<int*>(sd + NE);
+  unsigned int* smx = reinterpret_cast<unsigned int*>(sgm + NE);  // per-element max |x|, float bits
+  float* cw = reinterpret_cast<float*>(smx + NE);
+  int* ck = reinterpret_cast<int*>(cw + (size_t)R * J);
+  int* cn = ck + (size_t)R * J;
+  int* sslot = cn + R;
+  uint32_t* ssgn = reinterpret_cast<uint32_t*>(sslot + K);
+  const int nvalid = min(NE, S.nt - e0);
+  const int dg = S.sl_diag;
+  {
+    // every staging load is issued before the first of them is waited for
+    constexpr int W = sizeof(ElemDesc) / 4;
+    constexpr int DQ = (NE * W + NT - 1) / NT;
+    constexpr int TQ = 2;
+    const int32_t* src = reinterpret_cast<const int32_t*>(S.desc + e0);
+    int32_t dv[DQ];
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+      const int s2 = tid + NT * q;
+      dv[q] = (s2 < NE * W && s2 / W < nvalid) ? src[s2] : 0;
+    }
+    const int gmv = (op.constrained && tid < nvalid) ? S.gmask[e0 + tid] : 0;
+    float cwv[TQ];
+    int ckv[TQ];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) {
+      const int s2 = tid + NT * q;
+      cwv[q] = s2 < R * J ? S.ftc_w32[s2] : 0.0f;
+      ckv[q] = s2 < R * J ? S.ftc_key[s2] : 0;
+    }
+    const int cnv = tid < R ? S.ftc_n[tid] : 0;
+    const int slv = tid < K ? S.slot[tid] : 0;
+    const uint32_t sgv = tid < 6 ? S.sl_sgn[tid] : 0u;
+    int32_t* dst = reinterpret_cast<int32_t*>(sd);
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+      const int s2 = tid + NT * q;
+      if (s2 < NE * W) dst[s2] = dv[q];
+    }
+    if (tid < NE) { sgm[tid] = gmv; smx[tid] = 0u; }
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) {
+      const int s2 = tid + NT * q;
+      if (s2 < R * J) { cw[s2] = cwv[q]; ck[s2] = ckv[q]; }
+    }
+    for (int s2 = tid + NT * TQ; s2 < R * J; s2 += NT) { cw[s2] = S.ftc_w32[s2]; ck[s2] = S.ftc_key[s2]; }
+    if (tid < R) cn[tid] = cnv;
+    for (int s2 = tid + NT; s2 < R; s2 += NT) cn[s2] = S.ftc_n[s2];
+    if (tid < K) sslot[tid] = slv;
+    for (int s2 = tid + NT; s2 < K; s2 += NT) sslot[s2] = S.slot[s2];
+    if (tid < 6) ssgn[tid] = sgv;
+  }
+  __syncthreads();
+  // three groups of 8 modes per thread (NE * K32 / 8 = 768 groups: K32 <= 96
+  // with CT = 4, <= 192 with CT = 2), element fastest across the threads
+  constexpr int GQ = 3;
+  static_assert(NE * (CT == 4 ? 12 : 24) <= GQ * NT, "three groups of 8 modes per thread");
+  const int KBl = K32l / 8;
+  float xv[GQ][8];
+  auto coef = [&](int le, int k) -> float {
+    const ElemDesc& D = sd[le];
+    const int msk = sgm[le];
+    const int code = sslot[k];
+    const int kind = code & 3, geo = (code >> 2) & 7, sub = code >> 5;
+    if (kind == 0) return ((msk >> geo) & 1) ? 0.0f : x[D.v[geo]];
+    if (kind == 1) {
+      const int ed = D.e[geo];
+      const float v = ((msk >> (4 + geo)) & 1) ? 0.0f : x[(ed & 0x7fffffff) + sub];
+      return (ed < 0 && (sub & 1)) ? -v : v;
+    }
+    if (kind == 3) return ((msk >> 14) & 1) ? 0.0f : x[D.t + sub];
+    const int fo = D.fo[geo];
+    const bool fmask = (msk >> (10 + geo)) & 1;
+    if ((dg >> (fo & 7)) & 1) {
+      const float v = fmask ? 0.0f : x[D.f[geo] + sub];
+      return (sub < 32 && ((ssgn[fo & 7] >> sub) & 1)) ? -v : v;
+    }
+    const int row = (fo & 7) * S.ndg2 + sub;
+    float acc = 0.0f;
+    const int nterm = cn[row];
+    for (int j = 0; j < nterm; ++j) {
+      const int key = ck[row * J + j];
+      const int ie = key & 3, idx = key >> 2;
+      float w = cw[row * J + j];
+      float xq;
+      if (ie < 3) {
+        const int le2 = (fo >> (3 + 3 * ie)) & 7;
+        if (((fo >> (12 + ie)) & 1) && (idx & 1)) w = -w;
+        xq = ((msk >> (4 + le2)) & 1) ? 0.0f : x[(D.e[le2] & 0x7fffffff) + idx];
+      } else {
+        xq = fmask ? 0.0f : x[D.f[geo] + idx];
+      }
+      acc += xq * w;
+    }
+    return acc;
+  };
+#pragma unroll
+  for (int g = 0; g < GQ; ++g) {
+    const int gi = tid + NT * g, kb = gi / NE, le = gi - kb * NE;
+    float m = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = 8 * kb + t;
+      xv[g][t] = (kb < KBl && le < nvalid && k < Kl) ? coef(le, k) : 0.0f;
+      m = fmaxf(m, fabsf(xv[g][t]));
+    }
+    if (kb < KBl) atomicMax(&smx[le], __float_as_uint(m));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int g = 0; g < GQ; ++g) {
+    const int gi = tid + NT * g, kb = gi / NE, le = gi - kb * NE;
+    if (kb < KBl) {
+      const float m = __uint_as_float(smx[le]);
+      h8 hi, lo;
+      split8(xv[g], m > 0.0f ? 1.0f / m : 0.0f, hi, lo);
+      st8(xhh + ((size_t)kb * NE + le) * 8, hi);
+      st8(xhl + ((size_t)kb * NE + le) * 8, lo);
+    }
+  }
   __syncthreads();
 
   // ---- phase A: GEMM 1 + mixing; wave w owns m tiles w + WAVES tp, tp <
@@ -262,12 +249,6 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
     for (int a = 0; a < 3; ++a)
 #pragma unroll
       for (int c = 0; c < CT; ++c) acc[a][c] = f4{0, 0, 0, 0};
-    size_t xo[CT];
-#pragma unroll
-    for (int c = 0; c < CT; ++c) {
-      int e = e0 + 16 * c + lr;
-      xo[c] = ((size_t)(e >> 5) * KB * TILE + (e & 31)) * 8;
-    }
     const int kb1 = S.kbeg1h[t];
     for (int kk = kb1; kk < K32l; kk += 32) {
       h8 ah[3], al[3];
@@ -277,10 +258,11 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
         ah[a] = ld8(S.Dh1hi + o);
         al[a] = ld8(S.Dh1lo + o);
       }
-      const size_t ko = (size_t)(kk / 8 + lk) * TILE * 8;
+      const size_t ko = (size_t)(kk / 8 + lk) * NE * 8;
 #pragma unroll
       for (int c = 0; c < CT; ++c) {
-        h8 bh = ld8(xhi + xo[c] + ko), bl = ld8(xlo + xo[c] + ko);
+        const size_t o = ko + (size_t)(16 * c + lr) * 8;
+        h8 bh = ld8(xhh + o), bl = ld8(xhl + o);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           acc[a][c] = mma(ah[a], bh, acc[a][c]);
@@ -295,7 +277,7 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
       float g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0, g5 = 0;
       if (e < S.nt) {
         const float* g = S.ghat32 + 6 * (size_t)e;
-        float f = cst * xs[e];
+        float f = cst * __uint_as_float(smx[col]);
         g0 = f * g[0]; g1 = f * g[1]; g2 = f * g[2]; g3 = f * g[3]; g4 = f * g[4]; g5 = f * g[5];
       }
       float vmax = 0.0f;
@@ -601,30 +583,31 @@ bool slab_h_fits(const DevSpace& S) {
 
 size_t gemm_h_lds(const DevSpace& S) { return gemm_h_colmax_offset(S) + (size_t)16 * gemm_h_ct(S) * 4; }
 
+// bytes of the gather phase's LDS (split tile, descriptors, masks and
+// maxima, face table rows, slot codes, sign masks), below the maxima of V
+static size_t gather_lds_end(const DevSpace& S) {
+  const size_t NE = 16 * gemm_h_ct(S), R = 6 * (size_t)S.ndg2, J = S.ftc_J;
+  return 4 * (size_t)S.K32 * NE + NE * sizeof(ElemDesc) + 8 * NE + R * J * 8 + R * 4 + (size_t)S.K * 4 + 24;
+}
+
 bool mfma_h_supported(const DevSpace& S) {
-  return S.Dh1hi != nullptr && S.xhhi != nullptr && S.K <= 16 * RMAX && S.Mp <= 128 && S.K32 <= 192 &&
-         S.Mp / 16 <= WAVES * (4 / gemm_h_ct(S)) && gemm_h_lds(S) <= 160 * 1024;
+  return S.Dh1hi != nullptr && S.K <= 16 * RMAX && S.Mp <= 128 && S.K32 <= 192 &&
+         S.K32 <= (gemm_h_ct(S) == 4 ? 96 : 192) && S.Mp / 16 <= WAVES * (4 / gemm_h_ct(S)) &&
+         gemm_h_lds(S) <= 160 * 1024 && gather_lds_end(S) <= S.hlds_v;
 }
 
 void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, float* yl, bool slab, hipStream_t st) {
   if (slab && (op.Kl != 0 || !S.sl_meta)) throw std::logic_error("slab results need the full order and slab tables");
-  const int Kl = op.Kl > 0 ? op.Kl : S.K;
   const bool timed = op.timed();
-  int th = timed ? timing_begin(TK_GATHER32, st) : -1;
-  const size_t ftc = (size_t)6 * S.ndg2 * (S.ftc_J * 8 + 4) + (size_t)S.K * 4;
-  hipLaunchKernelGGL(k_gather_tiles_h, dim3((unsigned)S.ntile), dim3(256), (size_t)S.K32 * TILE * 4 + ftc, st, S, Kl, x,
-                     S.xhhi, S.xhlo, S.xhs, op.constrained);
-  timing_end(th, st);
   const int CT = gemm_h_ct(S);
   const int NE = 16 * CT;
   int nb = (S.nt + NE - 1) / NE;
-  th = timed ? timing_begin(TK_ELEM32, st) : -1;
+  // one kernel: gather, both GEMMs, mixing, slab epilogue (timed as the element kernel)
+  int th = timed ? timing_begin(TK_ELEM32, st) : -1;
   if (CT == 4)
-    hipLaunchKernelGGL(k_elem_gemm_h<4>, dim3(nb), dim3(NT), gemm_h_lds(S), st, S, op, S.xhhi, S.xhlo, S.xhs, yl,
-                       (int)slab);
+    hipLaunchKernelGGL(k_elem_gemm_h<4>, dim3(nb), dim3(NT), gemm_h_lds(S), st, S, op, x, yl, (int)slab);
   else
-    hipLaunchKernelGGL(k_elem_gemm_h<2>, dim3(nb), dim3(NT), gemm_h_lds(S), st, S, op, S.xhhi, S.xhlo, S.xhs, yl,
-                       (int)slab);
+    hipLaunchKernelGGL(k_elem_gemm_h<2>, dim3(nb), dim3(NT), gemm_h_lds(S), st, S, op, x, yl, (int)slab);
   timing_end(th, st);
 }
 

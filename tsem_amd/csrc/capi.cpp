@@ -426,10 +426,6 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   d.xt64 = sp->xt64.as<double>(); d.xt32 = sp->xt32.as<float>();
   d.Dh1hi = sp->Dh1hi.as<uint16_t>(); d.Dh1lo = sp->Dh1lo.as<uint16_t>();
   d.Dh2hi = sp->Dh2hi.as<uint16_t>(); d.Dh2lo = sp->Dh2lo.as<uint16_t>();
-  sp->xhhi.alloc((size_t)d.ntile * TILE * d.K32 * 2);
-  sp->xhlo.alloc((size_t)d.ntile * TILE * d.K32 * 2);
-  sp->xhs.alloc((size_t)d.ntile * TILE * 4);
-  d.xhhi = sp->xhhi.as<uint16_t>(); d.xhlo = sp->xhlo.as<uint16_t>(); d.xhs = sp->xhs.as<float>();
   {
     d.hlds_v = gemm_h_colmax_offset(d);
   }

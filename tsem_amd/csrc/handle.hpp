@@ -92,7 +92,7 @@ struct tsem_space {
   // boundary-condition quadrature and modal bases (bc.hip), offsets per table
   DevBuf bc_tab, bc_offd;
   std::vector<long> bc_off;
-  DevBuf Dh1hi, Dh1lo, Dh2hi, Dh2lo, xhhi, xhlo, xhs;  // fp16-split FP32 operator
+  DevBuf Dh1hi, Dh1lo, Dh2hi, Dh2lo;  // fp16-split FP32 operator factors
   tsem::Slab slab;                                      // entity-slab layout (host tables)
   // point location: element bounding boxes bucketed on a uniform grid
   // (tsem_point_values; dropped when the coordinates change)

@@ -89,15 +89,13 @@ struct DevSpace {
   // fp16-split FP32 operator: factors scaled by 1/dinv and split hi/lo,
   // Dh1[3][Mp][K32] (GEMM 1), Dh2[K16][3 Mh] (GEMM 2, D transposed, each
   // component padded to Mh = roundup(Kd, 32) rows); element vectors
-  // xh[tile][K32 / 8][32][8] hi/lo with per-element scales xhs
+  // (the element vectors are gathered and split inside the element kernel)
   int K32 = 0, Mh = 0, R2 = 0, R2g = 0;
   size_t hlds_v = 0;  // LDS byte offset of the per-element maxima
   float dinv = 1.0f;
   int kbeg1h[16] = {0};
   int kend2h[16] = {0};
   const uint16_t *Dh1hi = nullptr, *Dh1lo = nullptr, *Dh2hi = nullptr, *Dh2lo = nullptr;
-  uint16_t *xhhi = nullptr, *xhlo = nullptr;
-  float* xhs = nullptr;
   // tiled scratch for gathered element vectors
   double* xt64 = nullptr;
   float* xt32 = nullptr;

@@ -62,14 +62,7 @@ static size_t scratch_layout(DevSpace& d, const OpCoef& op, char* base, T** yl) 
   size_t off = align256(tiles * d.K * sizeof(T));  // results: at most one value per (element, mode)
   if (base) *yl = reinterpret_cast<T*>(base);
   if (sizeof(T) == 4 && op.mass == 0.0 && apply_path() == 0 && mfma_h_supported(d)) {
-    // fp16-split tile: hi and lo halves, one scale per element
-    const size_t h = tiles * d.K32 * 2;
-    if (base) {
-      d.xhhi = reinterpret_cast<uint16_t*>(base + off);
-      d.xhlo = reinterpret_cast<uint16_t*>(base + off + align256(h));
-      d.xhs = reinterpret_cast<float*>(base + off + 2 * align256(h));
-    }
-    off += 2 * align256(h) + align256(tiles * 4);
+    // the fp16-split kernel gathers into LDS: results only
   } else if (op.mass == 0.0 && (apply_path() == 0 || apply_path() == 2) && mfma_supported(d)) {
     const size_t t = tiles * d.Kp * sizeof(T);
     if (base) {
