@@ -231,6 +231,13 @@ int tsem_set_apply_path(int path);
  * off; kernel timing turns it off).  No counterpart. */
 int tsem_set_graphs(int enable);
 
+/* LOBPCG's p-multigrid preconditioner as one V-cycle over the block of
+ * residual columns per iteration, every kernel one launch over the columns
+ * (process wide; default on, TSEM_BLOCK_VCYCLE=0 turns it off, and it needs
+ * the fp16-split FP32 operator on every level).  Off: one V-cycle per
+ * column; the two give the same bits.  No counterpart. */
+int tsem_set_block_vcycle(int enable);
+
 /* Element results written as entity slabs (process wide; default on,
  * TSEM_SLAB=0 turns it off): the FP32 matrix-core element kernel stores each
  * element's contribution to each of its global DOFs, already through the

@@ -80,6 +80,39 @@ def test_lobpcg_deterministic(pencil):
     assert np.array_equal(l1, l2) and np.array_equal(X1.cpu().numpy(), X2.cpu().numpy())
 
 
+def test_lobpcg_vcycle_graph_matches_eager(pencil):
+    """The p-multigrid V-cycles of the preconditioner replayed from one
+    captured graph (hundreds of replays, eager block work between them) give
+    the same eigenpairs bit for bit as the same V-cycles launched eagerly."""
+    from tsem_amd import lib
+    psp, A, B, free, w = pencil
+    _say("lobpcg graphs on / off")
+    l1, X1, _ = psp.lobpcg(4, tol=1e-9, precond=4, seed=5)
+    lib.check(lib.load().tsem_set_graphs(0))
+    try:
+        l2, X2, _ = psp.lobpcg(4, tol=1e-9, precond=4, seed=5)
+    finally:
+        lib.check(lib.load().tsem_set_graphs(1))
+    assert np.array_equal(l1, l2) and np.array_equal(X1.cpu().numpy(), X2.cpu().numpy())
+
+
+def test_lobpcg_block_vcycle_matches_per_column(pencil):
+    """The preconditioner as one block V-cycle over all residual columns
+    (every kernel one launch over the columns) and as one V-cycle per
+    column: the same eigenpairs bit for bit."""
+    from tsem_amd import lib
+    psp, A, B, free, w = pencil
+    _say("lobpcg block V-cycle on / off")
+    l1, X1, s1 = psp.lobpcg(6, tol=1e-9, precond=4, seed=11)
+    lib.check(lib.load().tsem_set_block_vcycle(0))
+    try:
+        l2, X2, s2 = psp.lobpcg(6, tol=1e-9, precond=4, seed=11)
+    finally:
+        lib.check(lib.load().tsem_set_block_vcycle(1))
+    assert s1["iterations"] == s2["iterations"]
+    assert np.array_equal(l1, l2) and np.array_equal(X1.cpu().numpy(), X2.cpu().numpy())
+
+
 def test_inverse_power_lowest_eigenvalue(pencil):
     psp, A, B, free, w = pencil
     _say("inverse power")

@@ -1,4 +1,4 @@
-// Element operator on the matrix cores (gfx950 MFMA), in two kernels.
+// Element operator on the matrix cores (gfx950 MFMA).
 //
 // k_gather_tiles: element-local coefficients of 32-element tiles, written
 //   k-major inside the tile (xt[tile][k][32]) so the GEMM kernel streams them
@@ -15,6 +15,9 @@
 //   structurally zero blocks (about 30% of the dense work at p = 6).
 //   Y leaves through an LDS transpose, element-major (yl[e][i]), so the
 //   scatter reads each element's modes from contiguous memory.
+// k_mass_gemm: the mass term, Y = M X with the reference mass matrix, each
+//   element column scaled by its volume factor; stored, or added to the
+//   stiffness kernel's output.
 //
 // MFMA 16x16x4 operand layout (one 64-lane wave): lane l supplies
 // A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator value j of lane l is
@@ -349,6 +352,83 @@ __global__ void __launch_bounds__(NT) k_elem_gemm(DevSpace S, OpCoef op, const T
   }
 }
 
+// Mass term on the matrix cores: Y (K x NE) = M (K x K) X (K x NE) with the
+// reference mass matrix M = mfac^T mfac (Cp x Cp, zero padded), column e
+// scaled by mass 6|V_e|; the same operand layout as GEMM 1 above.  Wave w
+// owns the row tiles w + WAVES j.  With `accumulate` the result is added to
+// yl (the stiffness kernel has written it), else it is stored.
+template <typename T, int CT>
+__global__ void __launch_bounds__(NT) k_mass_gemm(DevSpace S, OpCoef op, const T* __restrict__ Mm,
+                                                  const T* __restrict__ xt, T* __restrict__ yl, int accumulate) {
+  using v4 = typename Mma<T>::v4;
+  constexpr int NE = 16 * CT;
+  constexpr int VS = NE + 1;
+  constexpr int RJ = (RMAX + WAVES - 1) / WAVES;
+  extern __shared__ unsigned char smem_raw[];
+  T* Y = reinterpret_cast<T*>(smem_raw);  // [Cp][VS]
+  const int K = S.K;
+  const int Kl = op.Kl > 0 ? op.Kl : K;
+  const int Cp = S.Cp, Kp = S.Kp;
+  const int Rt = (Kl + 15) >> 4;
+  const int Kpl = (Kl + 3) & ~3;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lr = lane & 15, lk = lane >> 4;
+  const int e0 = blockIdx.x * NE;
+  const T* xb[CT];
+  T scl[CT];
+#pragma unroll
+  for (int c = 0; c < CT; ++c) {
+    const int e = e0 + 16 * c + lr;
+    xb[c] = xt + (size_t)(e >> 5) * Kp * TILE + (size_t)lk * TILE + (e & 31);
+    scl[c] = e < S.nt ? (T)op.mass * ((sizeof(T) == 8) ? (T)S.sixv[e] : (T)S.sixv32[e]) : T(0);
+  }
+  v4 acc[RJ][CT];
+#pragma unroll
+  for (int j = 0; j < RJ; ++j)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[j][c] = v4{0, 0, 0, 0};
+  for (int k = 0; k < Kpl; k += 4) {
+    T b[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) b[c] = xb[c][(size_t)k * TILE];
+#pragma unroll
+    for (int j = 0; j < RJ; ++j) {
+      const int t = wave + WAVES * j;
+      if (t < Rt) {
+        const T a = Mm[(size_t)(16 * t + lr) * Cp + k + lk];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[j][c] = Mma<T>::op(a, b[c], acc[j][c]);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < RJ; ++j) {
+    const int t = wave + WAVES * j;
+    if (t < Rt) {
+#pragma unroll
+      for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int i = 16 * t + Mma<T>::row(lk, q);
+          Y[(size_t)i * VS + 16 * c + lr] = acc[j][c][q] * scl[c];
+        }
+    }
+  }
+  __syncthreads();
+  const int nval = min(NE, S.nt - e0);
+  for (int s = tid; s < nval * K; s += NT) {
+    const int le = s / K, i = s % K;
+    const T v = i < Kl ? Y[(size_t)i * VS + le] : T(0);
+    T* o = yl + (size_t)(e0 + le) * K + i;
+    *o = accumulate ? *o + v : v;
+  }
+}
+
+template <typename T, int CT>
+static size_t mass_lds(const DevSpace& S) {
+  return (size_t)S.Cp * (16 * CT + 1) * sizeof(T);
+}
+
 template <typename T, int CT>
 static size_t gemm_lds(const DevSpace& S) {
   size_t rows = (size_t)3 * S.Mp > (size_t)S.K ? (size_t)3 * S.Mp : (size_t)S.K;
@@ -356,12 +436,16 @@ static size_t gemm_lds(const DevSpace& S) {
 }
 
 bool mfma_supported(const DevSpace& S) {
-  return S.Dp64 != nullptr && S.xt64 != nullptr && S.K <= 16 * RMAX && gemm_lds<float, 4>(S) <= 160 * 1024 &&
-         gemm_lds<double, 2>(S) <= 160 * 1024;
+  return S.Dp64 != nullptr && S.Mm64 != nullptr && S.xt64 != nullptr && S.K <= 16 * RMAX &&
+         gemm_lds<float, 4>(S) <= 160 * 1024 && gemm_lds<double, 2>(S) <= 160 * 1024 &&
+         mass_lds<float, 4>(S) <= 160 * 1024 && mass_lds<double, 2>(S) <= 160 * 1024;
 }
 
+// stiffness (k_elem_gemm) and mass (k_mass_gemm) terms from one gather; an
+// operator with neither runs the stiffness kernel, which then writes zeros
 template <typename T, int CT>
-static void run(const DevSpace& S, const OpCoef& op, const T* Dp, T* xt, const T* x, T* yl, hipStream_t st) {
+static void run(const DevSpace& S, const OpCoef& op, const T* Dp, const T* Mm, T* xt, const T* x, T* yl,
+                hipStream_t st) {
   const int Kl = op.Kl > 0 ? op.Kl : S.K;
   const bool timed = op.timed();
   const int kind = sizeof(T) == 8 ? TK_GATHER64 : TK_GATHER32;
@@ -370,18 +454,22 @@ static void run(const DevSpace& S, const OpCoef& op, const T* Dp, T* xt, const T
   hipLaunchKernelGGL(k_gather_tiles<T>, dim3((unsigned)S.ntile), dim3(256), ftc, st, S, Kl, x, xt, op.constrained);
   timing_end(th, st);
   int nb = (S.nt + 16 * CT - 1) / (16 * CT);
+  const bool mass = op.mass != 0.0, stiff = op.stiff != 0.0 || !mass;
   th = timed ? timing_begin(kind + 1, st) : -1;
-  hipLaunchKernelGGL((k_elem_gemm<T, CT>), dim3(nb), dim3(NT), (gemm_lds<T, CT>(S)), st, S, op, Dp, xt, yl);
+  if (stiff) hipLaunchKernelGGL((k_elem_gemm<T, CT>), dim3(nb), dim3(NT), (gemm_lds<T, CT>(S)), st, S, op, Dp, xt, yl);
+  if (mass)
+    hipLaunchKernelGGL((k_mass_gemm<T, CT>), dim3(nb), dim3(NT), (mass_lds<T, CT>(S)), st, S, op, Mm, xt, yl,
+                       stiff ? 1 : 0);
   timing_end(th, st);
 }
 
 template <>
 void launch_apply_mfma<double>(const DevSpace& S, const OpCoef& op, const double* x, double* yl, hipStream_t st) {
-  run<double, 2>(S, op, S.Dp64, S.xt64, x, yl, st);
+  run<double, 2>(S, op, S.Dp64, S.Mm64, S.xt64, x, yl, st);
 }
 template <>
 void launch_apply_mfma<float>(const DevSpace& S, const OpCoef& op, const float* x, float* yl, hipStream_t st) {
-  run<float, 4>(S, op, S.Dp32, S.xt32, x, yl, st);
+  run<float, 4>(S, op, S.Dp32, S.Mm32, S.xt32, x, yl, st);
 }
 
 }  // namespace tsem

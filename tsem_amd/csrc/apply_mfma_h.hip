@@ -78,6 +78,8 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
   extern __shared__ unsigned char smem_raw[];
   uint16_t* Vh = reinterpret_cast<uint16_t*>(smem_raw);              // hi [NE][R2], lo [NE][R2]  (phase C)
   unsigned int* colmax = reinterpret_cast<unsigned int*>(smem_raw + S.hlds_v);  // [NE] float bits
+  x += (size_t)blockIdx.y * op.ldx;  // column of a block apply
+  yl += (size_t)blockIdx.y * op.ldl;
   const int K = S.K, K32 = S.K32, Mp = S.Mp, R2 = S.R2;
   const int Kl = op.Kl > 0 ? op.Kl : K;
   const int Kdl = op.Kdl > 0 ? op.Kdl : S.Kd;
@@ -604,10 +606,12 @@ void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, fl
   int nb = (S.nt + NE - 1) / NE;
   // one kernel: gather, both GEMMs, mixing, slab epilogue (timed as the element kernel)
   int th = timed ? timing_begin(TK_ELEM32, st) : -1;
+  if (op.ncol < 1) throw std::logic_error("block apply needs at least one column");
+  const dim3 grid(nb, op.ncol);
   if (CT == 4)
-    hipLaunchKernelGGL(k_elem_gemm_h<4>, dim3(nb), dim3(NT), gemm_h_lds(S), st, S, op, x, yl, (int)slab);
+    hipLaunchKernelGGL(k_elem_gemm_h<4>, grid, dim3(NT), gemm_h_lds(S), st, S, op, x, yl, (int)slab);
   else
-    hipLaunchKernelGGL(k_elem_gemm_h<2>, dim3(nb), dim3(NT), gemm_h_lds(S), st, S, op, x, yl, (int)slab);
+    hipLaunchKernelGGL(k_elem_gemm_h<2>, grid, dim3(NT), gemm_h_lds(S), st, S, op, x, yl, (int)slab);
   timing_end(th, st);
 }
 

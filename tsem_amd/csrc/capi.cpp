@@ -338,6 +338,19 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   }
   upload(sp->Dp64, Dp, st);
   upload(sp->Dp32, std::vector<float>(Dp.begin(), Dp.end()), st);
+  {
+    // reference mass matrix M = mfac^T mfac (the scalar kernel applies the
+    // two factors in turn), zero padded to Cp x Cp
+    std::vector<double> Mm((size_t)d.Cp * d.Cp, 0.0);
+    for (int i = 0; i < K; ++i)
+      for (int j = 0; j < K; ++j) {
+        double s = 0.0;
+        for (int m = 0; m < K; ++m) s += S.ref.mfac[(size_t)m * K + i] * S.ref.mfac[(size_t)m * K + j];
+        Mm[(size_t)i * d.Cp + j] = s;
+      }
+    upload(sp->Mm64, Mm, st);
+    upload(sp->Mm32, std::vector<float>(Mm.begin(), Mm.end()), st);
+  }
   // fp16-split factors: scaled by a power of two so that |D| < 1, each
   // entry as hi + lo fp16; GEMM 1 layout [3][Mp][K32], GEMM 2 layout
   // [K16][3 Mh] (transposed, components padded to Mh rows)
@@ -423,6 +436,7 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   d.sc_w = sp->sc_w.as<double>(); d.sc_w32 = sp->sc_w32.as<float>();
   d.diag = sp->diag_d.as<double>(); d.invdiag = sp->invdiag.as<double>(); d.invdiag32 = sp->invdiag32.as<float>();
   d.Dp64 = sp->Dp64.as<double>(); d.Dp32 = sp->Dp32.as<float>();
+  d.Mm64 = sp->Mm64.as<double>(); d.Mm32 = sp->Mm32.as<float>();
   d.xt64 = sp->xt64.as<double>(); d.xt32 = sp->xt32.as<float>();
   d.Dh1hi = sp->Dh1hi.as<uint16_t>(); d.Dh1lo = sp->Dh1lo.as<uint16_t>();
   d.Dh2hi = sp->Dh2hi.as<uint16_t>(); d.Dh2lo = sp->Dh2lo.as<uint16_t>();
@@ -799,6 +813,24 @@ int tsem_timing_read(double* out) {
   API_BEGIN
   if (!out) throw std::invalid_argument("null argument");
   timing_read(out);
+  API_END
+}
+
+}  // extern "C"
+
+bool tsem::graphs_enabled() { return g_pcg_graphs; }
+
+static bool g_block_vcycle = [] {
+  const char* v = getenv("TSEM_BLOCK_VCYCLE");
+  return !(v && v[0] == '0');
+}();
+bool tsem::block_vcycle_enabled() { return g_block_vcycle; }
+
+extern "C" {
+
+int tsem_set_block_vcycle(int enable) {
+  API_BEGIN
+  g_block_vcycle = enable != 0;
   API_END
 }
 

@@ -36,56 +36,115 @@ using namespace tsem;
 
 namespace {
 
-// Cyclic Jacobi eigenvalue method for a symmetric n x n matrix (row-major):
-// eigenvalues ascending in w, eigenvectors in the columns of V (row-major).
+// Symmetric eigenproblem (n x n, row-major): eigenvalues ascending in w,
+// eigenvectors in the columns of V (row-major).  Householder reduction to
+// tridiagonal form with the transforms accumulated, then the implicit QL
+// method with Wilkinson shifts on the tridiagonal matrix (Golub and Van Loan,
+// Matrix Computations, sections 8.3.1 and 8.3.3).  The accumulated transform
+// is kept transposed (row j = basis vector j) so every rotation and every
+// Householder update runs along contiguous rows.
 void sym_eig(int n, std::vector<double> A, std::vector<double>& w, std::vector<double>& V) {
-  std::vector<double> Q((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, diag = 0.0;
-    for (int i = 0; i < n; ++i) {
-      diag += A[(size_t)i * n + i] * A[(size_t)i * n + i];
-      for (int j = i + 1; j < n; ++j) off += A[(size_t)i * n + j] * A[(size_t)i * n + j];
+  std::vector<double> Zt((size_t)n * n, 0.0), d(n), e(n, 0.0), v(n), p(n), u(n);
+  for (int i = 0; i < n; ++i) Zt[(size_t)i * n + i] = 1.0;
+  for (int k = 0; k + 2 < n; ++k) {
+    // H = I - beta v v^T maps A[k+1:, k] onto a multiple of e_{k+1}
+    double s = 0.0;
+    for (int i = k + 1; i < n; ++i) s += A[(size_t)i * n + k] * A[(size_t)i * n + k];
+    const double x0 = A[(size_t)(k + 1) * n + k];
+    const double tail = s - x0 * x0;
+    if (tail == 0.0) continue;  // column already tridiagonal
+    const double alpha = x0 > 0 ? -std::sqrt(s) : std::sqrt(s);
+    for (int i = k + 1; i < n; ++i) v[i] = A[(size_t)i * n + k];
+    v[k + 1] = x0 - alpha;
+    const double beta = 2.0 / (tail + v[k + 1] * v[k + 1]);
+    // trailing block <- H A H = A - v q^T - q v^T with p = beta A v and
+    // q = p - (beta p.v / 2) v
+    double pv = 0.0;
+    for (int i = k + 1; i < n; ++i) {
+      const double* Ai = &A[(size_t)i * n];
+      double acc = 0.0;
+      for (int j = k + 1; j < n; ++j) acc += Ai[j] * v[j];
+      p[i] = beta * acc;
+      pv += p[i] * v[i];
     }
-    if (off <= 1e-32 * diag || off == 0.0) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        double apq = A[(size_t)p * n + q];
-        if (apq == 0.0) continue;
-        double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
-        if (std::fabs(apq) < 1e-300) continue;
-        // rotation that zeroes A[p][q]: tan(phi) = t, the smaller root of
-        // t^2 + 2 theta t - 1 = 0 with theta = (aqq - app) / (2 apq)
-        double theta = (aqq - app) / (2.0 * apq);
-        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < n; ++k) {  // columns p, q
-          double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
-          A[(size_t)k * n + p] = c * akp - s * akq;
-          A[(size_t)k * n + q] = s * akp + c * akq;
+    const double h = 0.5 * beta * pv;
+    for (int i = k + 1; i < n; ++i) p[i] -= h * v[i];
+    for (int i = k + 1; i < n; ++i) {
+      double* Ai = &A[(size_t)i * n];
+      for (int j = k + 1; j < n; ++j) Ai[j] -= v[i] * p[j] + p[i] * v[j];
+    }
+    A[(size_t)(k + 1) * n + k] = A[(size_t)k * n + k + 1] = alpha;
+    for (int i = k + 2; i < n; ++i) A[(size_t)i * n + k] = A[(size_t)k * n + i] = 0.0;
+    // accumulated transform: Zt[k+1:, :] <- H Zt[k+1:, :]
+    std::fill(u.begin(), u.end(), 0.0);
+    for (int i = k + 1; i < n; ++i) {
+      const double* Zi = &Zt[(size_t)i * n];
+      for (int c = 0; c < n; ++c) u[c] += v[i] * Zi[c];
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = beta * v[i];
+      double* Zi = &Zt[(size_t)i * n];
+      for (int c = 0; c < n; ++c) Zi[c] -= f * u[c];
+    }
+  }
+  // tridiagonal: d on the diagonal, e[i] couples rows i and i + 1
+  for (int i = 0; i < n; ++i) d[i] = A[(size_t)i * n + i];
+  for (int i = 0; i + 1 < n; ++i) e[i] = A[(size_t)(i + 1) * n + i];
+  const double eps = 2.220446049250313e-16;
+  for (int l = 0; l < n; ++l) {
+    for (int iter = 0; iter < 64; ++iter) {
+      // the first negligible coupling at or after l closes the active block
+      int m = l;
+      for (; m + 1 < n; ++m)
+        if (std::fabs(e[m]) <= eps * (std::fabs(d[m]) + std::fabs(d[m + 1]))) break;
+      if (m == l) break;
+      // shift: the eigenvalue of the leading 2 x 2 block nearer d[l]
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = std::hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
+      double s = 1.0, c = 1.0, shift = 0.0;
+      bool deflated = false;
+      for (int i = m - 1; i >= l; --i) {
+        // Givens rotation of rows i, i + 1 chasing the bulge upwards
+        const double f = s * e[i], b = c * e[i];
+        r = std::hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {  // the coupling underflowed: split the block here
+          d[i + 1] -= shift;
+          e[m] = 0.0;
+          deflated = true;
+          break;
         }
-        for (int k = 0; k < n; ++k) {  // rows p, q
-          double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
-          A[(size_t)p * n + k] = c * apk - s * aqk;
-          A[(size_t)q * n + k] = s * apk + c * aqk;
-        }
-        A[(size_t)p * n + q] = A[(size_t)q * n + p] = 0.0;
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - shift;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        shift = s * r;
+        d[i + 1] = g + shift;
+        g = c * r - b;
+        double* Za = &Zt[(size_t)i * n];
+        double* Zb = &Zt[(size_t)(i + 1) * n];
         for (int k = 0; k < n; ++k) {
-          double qkp = Q[(size_t)k * n + p], qkq = Q[(size_t)k * n + q];
-          Q[(size_t)k * n + p] = c * qkp - s * qkq;
-          Q[(size_t)k * n + q] = s * qkp + c * qkq;
+          const double za = Za[k], zb = Zb[k];
+          Zb[k] = s * za + c * zb;
+          Za[k] = c * za - s * zb;
         }
       }
+      if (deflated) continue;
+      d[l] -= shift;
+      e[l] = g;
+      e[m] = 0.0;
+    }
   }
   std::vector<int> order(n);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return A[(size_t)a * n + a] < A[(size_t)b * n + b]; });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return d[a] < d[b]; });
   w.resize(n);
   V.assign((size_t)n * n, 0.0);
   for (int c = 0; c < n; ++c) {
-    w[c] = A[(size_t)order[c] * n + order[c]];
-    for (int k = 0; k < n; ++k) V[(size_t)k * n + c] = Q[(size_t)k * n + order[c]];
+    w[c] = d[order[c]];
+    const double* Zc = &Zt[(size_t)order[c] * n];
+    for (int k = 0; k < n; ++k) V[(size_t)k * n + c] = Zc[k];
   }
 }
 
@@ -220,19 +279,18 @@ int tsem_lobpcg(tsem_space* sp, int m, double* X, double* lambda, const tsem_lob
   // (np columns) at column m, W (nw columns: residuals of the pairs not yet
   // converged, preconditioned, B-orthogonalized against X and P and
   // orthonormalized) right after P
-  DevBuf S, AS, BS, R, tmp, part, gram, dlam, dC, r32, z32;
+  DevBuf S, AS, BS, R, part, gram, dlam, dC, r32, z32, dscale;
   S.alloc(3 * nm * 8); AS.alloc(3 * nm * 8); BS.alloc(3 * nm * 8);
   R.alloc(nm * 8);
-  tmp.alloc(2 * nm * 8);
   part.alloc((size_t)gram_blocks(n) * 96 * 96 * 8);
   gram.alloc(96 * 96 * 8);
   dlam.alloc(32 * 8);
-  dC.alloc(96 * 32 * 8);
+  dC.alloc(96 * 64 * 8);
+  dscale.alloc(2 * 32 * 8);
   double* base[3] = {S.as<double>(), AS.as<double>(), BS.as<double>()};
   double* Sx = base[0];
   double* Ax = base[1];
   double* Bx = base[2];
-  double* T = tmp.as<double>();
   const uint8_t* cm = sp->d.cmask;
   int applies = 0;
 
@@ -248,13 +306,11 @@ int tsem_lobpcg(tsem_space* sp, int m, double* X, double* lambda, const tsem_lob
     HIPCHK(hipMemcpyAsync(dC.p, C.data(), C.size() * 8, hipMemcpyHostToDevice, st));
     return dC.as<double>();
   };
-  // columns [c0, c0 + k) of S, AS, BS <- the same columns times Tm (k x k2)
+  // columns [c0, c0 + k2) of S, AS, BS <- columns [c0, c0 + k) times Tm
+  // (k x k2), in place (the combination reads a row whole before writing it)
   auto transform = [&](int c0, int k, int k2, const std::vector<double>& Tm) {
     const double* dt = upload_coef(Tm);
-    for (int b = 0; b < 3; ++b) {
-      launch_combine(n, k, k2, col(b, c0), n, dt, T, n, 1.0, 0.0, st);
-      if (k2 > 0) HIPCHK(hipMemcpyAsync(col(b, c0), T, (size_t)n * k2 * 8, hipMemcpyDeviceToDevice, st));
-    }
+    for (int b = 0; b < 3; ++b) launch_combine(n, k, k2, col(b, c0), n, dt, col(b, c0), n, 1.0, 0.0, st);
   };
   // columns [w0, w0 + nw) -= columns [c0, c0 + k) Cm, Cm = ([c0..]^T B [w0..]) (k x nw)
   auto orthogonalize = [&](int c0, int k, int w0, int nw) {
@@ -317,6 +373,20 @@ int tsem_lobpcg(tsem_space* sp, int m, double* X, double* lambda, const tsem_lob
     double* Wp = col(0, w0);
     if (o->precond == 1) {
       launch_block_mask(n, nw, cm, sp->d.invdiag, Wp, st);
+    } else if (o->precond == 4 && nw > 1 && block_vcycle_enabled() && sp->pmg->block_supported()) {
+      // every column r_k / |r_k| through one block V-cycle, scaled back
+      r32.alloc((size_t)n * nw * 4);
+      z32.alloc((size_t)n * nw * 4);
+      std::vector<double> sc(2 * (size_t)nw);
+      for (int k = 0; k < nw; ++k) {
+        sc[k] = wnorm[k] > 0 ? 1.0 / wnorm[k] : 0.0;
+        sc[nw + k] = wnorm[k] > 0 ? wnorm[k] : 0.0;
+      }
+      HIPCHK(hipMemcpyAsync(dscale.p, sc.data(), sc.size() * 8, hipMemcpyHostToDevice, st));
+      launch_cast_cols(n, nw, Wp, dscale.as<double>(), r32.as<float>(), st);
+      sp->pmg->vcycle_replay<float>(r32.as<float>(), z32.as<float>(), o->smooth_steps, st, nw);
+      launch_widen_cols(n, nw, z32.as<float>(), dscale.as<double>() + nw, Wp, st);
+      launch_block_mask(n, nw, cm, nullptr, Wp, st);
     } else if (o->precond == 4) {
       r32.alloc((size_t)n * 4);
       z32.alloc((size_t)n * 4);
@@ -325,9 +395,8 @@ int tsem_lobpcg(tsem_space* sp, int m, double* X, double* lambda, const tsem_lob
         if (!(nr > 0)) continue;
         double* w = Wp + (size_t)k * n;
         launch_cast_f64_f32(n, w, 1.0 / nr, r32.as<float>(), st);
-        sp->pmg->vcycle<float>(r32.as<float>(), z32.as<float>(), o->smooth_steps, st);
-        HIPCHK(hipMemsetAsync(w, 0, (size_t)n * 8, st));
-        launch_add_f32_f64(n, z32.as<float>(), nr, w, st);
+        sp->pmg->vcycle_replay<float>(r32.as<float>(), z32.as<float>(), o->smooth_steps, st);
+        launch_cast_f32_f64(n, z32.as<float>(), nr, w, st);
       }
       launch_block_mask(n, nw, cm, nullptr, Wp, st);
     }
@@ -348,14 +417,17 @@ int tsem_lobpcg(tsem_space* sp, int m, double* X, double* lambda, const tsem_lob
     gram_host(p, p, Sx, Ax, GA);
     gram_host(p, p, Sx, Bx, GB);
     if (!rayleigh_ritz(p, GA, GB, m, 1e-13, C, lam)) throw std::runtime_error("LOBPCG trial space collapsed");
-    // X <- [X P W] C, P <- [P W] C[m:], then orthonormalize P
-    const double* dc = upload_coef(C);
-    for (int b = 0; b < 3; ++b) {
-      launch_combine(n, p, m, base[b], n, dc, T, n, 1.0, 0.0, st);
-      if (p > m) launch_combine(n, p - m, m, col(b, m), n, dc + (size_t)m * m, T + nm, n, 1.0, 0.0, st);
-      HIPCHK(hipMemcpyAsync(base[b], T, nm * 8, hipMemcpyDeviceToDevice, st));
-      if (p > m) HIPCHK(hipMemcpyAsync(col(b, m), T + nm, nm * 8, hipMemcpyDeviceToDevice, st));
-    }
+    // X <- [X P W] C and P <- [P W] C[m:] as one in-place combination with
+    // the coefficients [C | (0; C[m:])] (p x 2m), then orthonormalize P
+    const int m2 = p > m ? 2 * m : m;
+    std::vector<double> C2((size_t)p * m2, 0.0);
+    for (int i = 0; i < p; ++i)
+      for (int j = 0; j < m; ++j) {
+        C2[(size_t)i * m2 + j] = C[(size_t)i * m + j];
+        if (p > m && i >= m) C2[(size_t)i * m2 + m + j] = C[(size_t)i * m + j];
+      }
+    const double* dc = upload_coef(C2);
+    for (int b = 0; b < 3; ++b) launch_combine(n, p, m2, base[b], n, dc, base[b], n, 1.0, 0.0, st);
     np = 0;
     if (p > m) {
       gram_host(m, m, col(0, m), col(2, m), GB);

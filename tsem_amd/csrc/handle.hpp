@@ -84,7 +84,7 @@ struct tsem_space {
   // device storage
   DevBuf ftc_n, ftc_key, ftc_w, ftc_w32;
   DevBuf desc, slot, ft_ptr, ft_key, ft_w, ghat, ghat32, sixv, sixv32, dT, dN, dT32, dN32, mT, mN, mT32, mN32;
-  DevBuf sc_ptr, sc_code, sc_w, sc_w32, diag_d, invdiag, invdiag32, cmask_d, xe, pts3, wphi, Dp64, Dp32;
+  DevBuf sc_ptr, sc_code, sc_w, sc_w32, diag_d, invdiag, invdiag32, cmask_d, xe, pts3, wphi, Dp64, Dp32, Mm64, Mm32;
   DevBuf yl64, yl32, bl, partial, scal, xt64, xt32, gmask;
   DevBuf rhs_part;  // point-range partial sums of the load-vector GEMM
   // quadrature-point work: basis at the points, weights, potential, scratch

@@ -245,8 +245,11 @@ __device__ inline double block_sum(double v, double* red) {
 template <typename T>
 __global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict__ yl, T* __restrict__ y,
                                                  const T* __restrict__ xin, int constrained,
-                                                 double* __restrict__ partial) {
+                                                 double* __restrict__ partial, long ldl, long ldy) {
 #pragma clang fp contract(off)
+  yl += (size_t)blockIdx.y * ldl;  // column of a block apply
+  y += (size_t)blockIdx.y * ldy;
+  if (xin) xin += (size_t)blockIdx.y * ldy;
   __shared__ double red[BLK];
   constexpr int U = 8;
   const T* __restrict__ wt = (sizeof(T) == 8) ? (const T*)S.sc_w : (const T*)S.sc_w32;
@@ -301,8 +304,11 @@ static_assert(SCH == SLAB_ROWS, "one slab chunk per scatter block");
 template <typename T>
 __global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __restrict__ yl, T* __restrict__ y,
                                                         const T* __restrict__ xin, int constrained,
-                                                        double* __restrict__ partial) {
+                                                        double* __restrict__ partial, long ldl, long ldy) {
 #pragma clang fp contract(off)
+  yl += (size_t)blockIdx.y * ldl;  // column of a block apply
+  y += (size_t)blockIdx.y * ldy;
+  if (xin) xin += (size_t)blockIdx.y * ldy;
   extern __shared__ unsigned char scat_raw[];
   T* vals = reinterpret_cast<T*>(scat_raw);
   __shared__ double red[BLK];
@@ -374,7 +380,10 @@ constexpr int SU = 4;   // span entries per thread and chunk: spans <= SU * BLK
 template <typename T>
 __global__ void __launch_bounds__(BLK) k_scatter_slab(DevSpace S, const T* __restrict__ sl, T* __restrict__ y,
                                                       const T* __restrict__ xin, int constrained,
-                                                      double* __restrict__ partial) {
+                                                      double* __restrict__ partial, long ldl, long ldy) {
+  sl += (size_t)blockIdx.y * ldl;  // column of a block apply
+  y += (size_t)blockIdx.y * ldy;
+  if (xin) xin += (size_t)blockIdx.y * ldy;
   extern __shared__ unsigned char scat_raw[];
   T* vals = reinterpret_cast<T*>(scat_raw);  // [SP][sl_chunk_max]
   __shared__ double red[BLK];
@@ -465,23 +474,27 @@ static bool g_slab = slab_from_env();
 void set_slab(bool on) { g_slab = on; }
 bool slab_enabled() { return g_slab; }
 
+// ncol > 1: a block of columns (element results ldl apart, DOF vectors ldy
+// apart), with no partial sums
 template <typename T>
 static void slab_scatter(const DevSpace& S, const T* sl, T* y, const T* x, int constrained, double* partial,
-                         hipStream_t st) {
-  hipLaunchKernelGGL(k_scatter_slab<T>, dim3(reduce_blocks(S.ndof)), dim3(BLK),
-                     (size_t)SP * S.sl_chunk_max * sizeof(T), st, S, sl, y, x, constrained, partial);
+                         hipStream_t st, int ncol = 1, long ldl = 0, long ldy = 0) {
+  if (ncol > 1 && partial) throw std::logic_error("block scatter takes no partial sums");
+  hipLaunchKernelGGL(k_scatter_slab<T>, dim3(reduce_blocks(S.ndof), ncol), dim3(BLK),
+                     (size_t)SP * S.sl_chunk_max * sizeof(T), st, S, sl, y, x, constrained, partial, ldl, ldy);
 }
 
 // the chunked scatter when a chunk's terms fit in LDS, else the row kernel
 template <typename T>
 static void scatter_dispatch(const DevSpace& S, const T* yl, T* y, const T* x, int constrained, double* partial,
-                             hipStream_t st) {
-  int ng = reduce_blocks(S.ndof);
+                             hipStream_t st, int ncol = 1, long ldl = 0, long ldy = 0) {
+  if (ncol > 1 && partial) throw std::logic_error("block scatter takes no partial sums");
+  const dim3 grid(reduce_blocks(S.ndof), ncol);
   size_t lds = (size_t)S.sc_chunk_max * sizeof(T);
   if (S.sc_chunk_max > 0 && lds <= 64 * 1024)
-    hipLaunchKernelGGL(k_scatter_chunks<T>, dim3(ng), dim3(BLK), lds, st, S, yl, y, x, constrained, partial);
+    hipLaunchKernelGGL(k_scatter_chunks<T>, grid, dim3(BLK), lds, st, S, yl, y, x, constrained, partial, ldl, ldy);
   else
-    hipLaunchKernelGGL(k_scatter<T>, dim3(ng), dim3(BLK), 0, st, S, yl, y, x, constrained, partial);
+    hipLaunchKernelGGL(k_scatter<T>, grid, dim3(BLK), 0, st, S, yl, y, x, constrained, partial, ldl, ldy);
 }
 
 // ---- optional per-kernel timing with HIP events on the launching stream ----
@@ -535,18 +548,32 @@ static int apply_path_from_env() {
 }
 static int g_apply_path = apply_path_from_env();
 void set_apply_path(int path) { g_apply_path = path; }
+bool block_apply_supported(const DevSpace& S, const OpCoef& op) {
+  return op.ncol >= 1 && op.mass == 0.0 && op.pot == 0.0 && g_apply_path == 0 && mfma_h_supported(S);
+}
 int apply_path() { return g_apply_path; }
 
 template <typename T>
 static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, double* partial, hipStream_t st) {
   const int kind = (sizeof(T) == 8) ? TK_GATHER64 : TK_GATHER32;
+  if (op.ncol != 1) {
+    // a block of columns: the fp16-split element kernel and the scatter,
+    // each one launch over (blocks, columns)
+    if (!(sizeof(T) == 4 && block_apply_supported(S, op)))
+      throw std::logic_error("block apply needs the fp16-split FP32 path without mass or potential terms");
+    const bool slab = g_slab && S.sl_meta && op.Kl == 0 && slab_scatter_fits(S) && slab_h_fits(S);
+    launch_apply_mfma_h(S, op, (const float*)x, (float*)yl, slab, st);
+    if (slab) slab_scatter<T>(S, yl, y, x, op.constrained, partial, st, op.ncol, op.ldl, op.ldx);
+    else scatter_dispatch<T>(S, yl, y, x, op.constrained, partial, st, op.ncol, op.ldl, op.ldx);
+    return;
+  }
   bool slab = false;
   if (sizeof(T) == 4 && op.mass == 0.0 && g_apply_path == 0 && mfma_h_supported(S)) {
     // entity-slab results when the element kernel's output goes straight to
     // the scatter (full order, no potential term added to it)
     slab = g_slab && S.sl_meta && op.Kl == 0 && op.pot == 0.0 && slab_scatter_fits(S) && slab_h_fits(S);
     launch_apply_mfma_h(S, op, (const float*)x, (float*)yl, slab, st);
-  } else if (op.mass == 0.0 && (g_apply_path == 0 || g_apply_path == 2) && mfma_supported(S)) {
+  } else if ((g_apply_path == 0 || g_apply_path == 2) && mfma_supported(S)) {
     launch_apply_mfma<T>(S, op, x, yl, st);
   } else {
     constexpr int NE = 8;
@@ -862,6 +889,13 @@ __global__ void k_addf(int n, const float* __restrict__ x, double s, double* __r
 }
 void launch_add_f32_f64(int n, const float* x, double s, double* y, hipStream_t st) {
   hipLaunchKernelGGL(k_addf, dim3(grid1(n)), dim3(BLK), 0, st, n, x, s, y);
+}
+__global__ void k_widen(int n, const float* __restrict__ x, double s, double* __restrict__ y) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i < n) y[i] = s * (double)x[i];
+}
+void launch_cast_f32_f64(int n, const float* x, double s, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_widen, dim3(grid1(n)), dim3(BLK), 0, st, n, x, s, y);
 }
 
 // partial sums of |r32 + s r64|_2^2: how far an FP32 recursive residual has

@@ -83,6 +83,10 @@ struct DevSpace {
   int Mp = 0, Cp = 0, Kp = 0, ntile = 0;
   const double* Dp64 = nullptr;
   const float* Dp32 = nullptr;
+  // reference mass matrix mfac^T mfac, Cp x Cp (zero padded), for the MFMA
+  // mass kernel
+  const double* Mm64 = nullptr;
+  const float* Mm32 = nullptr;
   // per 16-row tile: first column of GEMM 1, last row (exclusive) of GEMM 2
   int kbeg1[16] = {0};
   int kend2[16] = {0};
@@ -123,6 +127,11 @@ struct OpCoef {
   int constrained = 0;   // honour the Dirichlet elimination
   int Kl = 0, Kdl = 0;   // leading modes used; 0 = all
   int level = 0;         // p-multigrid level the apply belongs to (0: the space itself)
+  // a block of ncol vectors in one launch (grid y = column): column c of x
+  // and y starts at c ldx, its element results at c ldl.  Only the
+  // fp16-split FP32 path takes ncol > 1 (the p-multigrid block V-cycle)
+  int ncol = 1;
+  long ldx = 0, ldl = 0;
   // applies counted by the per-kernel timing: the space's own full-order operator
   bool timed() const { return Kl == 0 && level == 0; }
 };
@@ -171,6 +180,14 @@ void timing_end(int handle, hipStream_t st);
 // 0: auto (MFMA when supported), 1: force the scalar element kernel
 void set_apply_path(int path);
 int apply_path();
+// the FP32 operator on a block of columns in one launch per kernel
+// (OpCoef::ncol): fp16-split path, stiffness only, default apply path
+bool block_apply_supported(const DevSpace& S, const OpCoef& op);
+// captured-graph replay of PCG chunks and V-cycles on (tsem_set_graphs)
+bool graphs_enabled();
+// the eigen solvers' preconditioner as one block V-cycle per iteration
+// (tsem_set_block_vcycle)
+bool block_vcycle_enabled();
 template <typename T>
 void launch_scatter(const DevSpace& S, const T* yl, T* y, hipStream_t st);
 
@@ -204,6 +221,8 @@ template <typename T>
 void launch_dot(int n, const T* a, const T* b, double* partial, hipStream_t st);
 void launch_cast_f64_f32(int n, const double* x, double scale, float* y, hipStream_t st);
 void launch_add_f32_f64(int n, const float* x, double scale, double* y, hipStream_t st);
+// y = scale x
+void launch_cast_f32_f64(int n, const float* x, double scale, double* y, hipStream_t st);
 // partial[b] of |r32 + s r64|_2^2 over reduce_blocks(n) blocks
 void launch_gap(int n, const float* r32, const double* r64, double s, double* partial, hipStream_t st);
 void launch_residual(const DevSpace& S, int n, const double* b, const double* Ax, double* r, double* partial, hipStream_t st);
@@ -225,6 +244,9 @@ void launch_gram(int n, int p, int q, const double* U, long ldu, const double* V
 // Y = alpha S C + beta Y with C row-major p x m on device (p <= 96, m <= 32)
 void launch_combine(int n, int p, int m, const double* S, long lds, const double* C, double* Y, long ldy,
                     double alpha, double beta, hipStream_t st);
+// per-column scaled casts of blocks (ncol columns n apart, scales on the device)
+void launch_cast_cols(int n, int ncol, const double* X, const double* s, float* Y, hipStream_t st);
+void launch_widen_cols(int n, int ncol, const float* Z, const double* s, double* W, hipStream_t st);
 // R[:, j] = AX[:, j] - lam[j] BX[:, j] (lam on device), zero on eliminated rows
 void launch_block_resid(int n, int m, const double* AX, const double* BX, const double* lam, const uint8_t* cmask,
                         double* R, hipStream_t st);

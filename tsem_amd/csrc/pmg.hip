@@ -18,11 +18,17 @@ inline int nblk(long n) { return (int)((n + BLK - 1) / BLK); }
 
 // Chebyshev step: d = c1 d + c2 Dinv (b - Ax); x += d.  Ax null: x = 0 before
 // the step (b - Ax = b), and with c1 = 0 neither d nor x is read
+// (every kernel here takes column blockIdx.y of a block of vectors, n apart)
 template <typename T>
 __global__ void k_cheb(int n, const T* __restrict__ b, const T* __restrict__ Ax, const T* __restrict__ inv, T c1,
                        T c2, T* __restrict__ dv, T* __restrict__ x) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
+  const size_t o = (size_t)blockIdx.y * n;
+  b += o;
+  if (Ax) Ax += o;
+  dv += o;
+  x += o;
   const T r = Ax ? b[i] - Ax[i] : b[i];
   T dn = c2 * inv[i] * r;
   if (c1 != T(0)) dn += c1 * dv[i];
@@ -32,17 +38,20 @@ __global__ void k_cheb(int n, const T* __restrict__ b, const T* __restrict__ Ax,
 
 // coarse right-hand side: bc[i] = bf[m[i]] - Axf[m[i]] (restricted residual)
 template <typename T>
-__global__ void k_restrict_res(int nc, const int32_t* __restrict__ m, const T* __restrict__ bf,
+__global__ void k_restrict_res(int nc, long nf, const int32_t* __restrict__ m, const T* __restrict__ bf,
                                const T* __restrict__ axf, T* __restrict__ bc) {
   const int i = blockIdx.x * BLK + threadIdx.x;
-  if (i < nc) bc[i] = bf[m[i]] - axf[m[i]];
+  const size_t of = (size_t)blockIdx.y * nf, oc = (size_t)blockIdx.y * nc;
+  if (i < nc) bc[oc + i] = bf[of + m[i]] - axf[of + m[i]];
 }
 
 // xf[m[i]] += xc[i] (zero extension of the coarse correction; m injective)
 template <typename T>
-__global__ void k_prolong_add(int nc, const int32_t* __restrict__ m, const T* __restrict__ xc, T* __restrict__ xf) {
+__global__ void k_prolong_add(int nc, long nf, const int32_t* __restrict__ m, const T* __restrict__ xc,
+                              T* __restrict__ xf) {
   const int i = blockIdx.x * BLK + threadIdx.x;
-  if (i < nc) xf[m[i]] += xc[i];
+  const size_t of = (size_t)blockIdx.y * nf, oc = (size_t)blockIdx.y * nc;
+  if (i < nc) xf[of + m[i]] += xc[oc + i];
 }
 
 // power iteration for lambda_max(D^-1 A) in the D inner product: partials of
@@ -88,9 +97,10 @@ __global__ void k_csr_spmv(int n, const int32_t* __restrict__ rs, const int32_t*
                            const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y) {
   const int i = blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
+  const size_t o = (size_t)blockIdx.y * n;
   T acc = T(0);
-  for (int k = rs[i]; k < rs[i + 1]; ++k) acc += val[k] * x[col[k]];
-  y[i] = acc;
+  for (int k = rs[i]; k < rs[i + 1]; ++k) acc += val[k] * x[o + col[k]];
+  y[o + i] = acc;
 }
 
 template <typename T>
@@ -288,6 +298,15 @@ Pmg::Pmg(tsem_space* fsp, hipStream_t st) : fine(fsp) {
 Pmg::~Pmg() { release(); }
 
 void Pmg::release() {
+  if (vexec) (void)hipGraphExecDestroy(vexec);
+  vexec = nullptr;
+  vkey.clear();
+  for (auto* v : {&bx, &bb, &br, &bd, &byl})
+    for (void*& p : *v) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+    }
+  bcap = 0;
   for (auto* v : {&vx, &vb, &vr, &vd})
     for (void*& p : *v) {
       if (p) (void)hipFree(p);
@@ -307,12 +326,55 @@ void Pmg::release() {
   }
 }
 
+long Pmg::yl_cols(int l) const {
+  const tsem_space* s = l == 0 ? fine : sub[l - 1];
+  return (long)(s->yl32.n / sizeof(float));
+}
+
+bool Pmg::block_supported() const {
+  for (int l = 0; l < levels(); ++l) {
+    if (l == csr_level) continue;
+    OpCoef op;
+    op.constrained = 1;
+    op.level = l;
+    op.ncol = 2;
+    if (!block_apply_supported(dev(l), op)) return false;
+  }
+  return true;
+}
+
+void Pmg::block_reserve(int ncol) {
+  if (ncol <= bcap) return;
+  // the captured V-cycle may hold the buffers about to be replaced
+  if (vexec) (void)hipGraphExecDestroy(vexec);
+  vexec = nullptr;
+  vkey.clear();
+  const int L = levels();
+  for (auto* v : {&bx, &bb, &br, &bd, &byl}) {
+    for (void*& p : *v)
+      if (p) (void)hipFree(p);
+    v->assign(L, nullptr);
+  }
+  bcap = 0;
+  for (int l = 0; l < L; ++l) {
+    const size_t b = (size_t)ncol * ndof_level[l] * sizeof(float);
+    HIPCHK(hipMalloc(&br[l], b));
+    HIPCHK(hipMalloc(&bd[l], b));
+    if (l > 0) {
+      HIPCHK(hipMalloc(&bx[l], b));
+      HIPCHK(hipMalloc(&bb[l], b));
+    }
+    if (l != csr_level) HIPCHK(hipMalloc(&byl[l], (size_t)ncol * yl_cols(l) * sizeof(float)));
+  }
+  bcap = ncol;
+}
+
 template <typename T>
-void Pmg::apply_level(int l, const T* x, T* y, hipStream_t st) {
+void Pmg::apply_level(int l, const T* x, T* y, hipStream_t st, int ncol) {
+  const int n = (int)ndof_level[l];
   if (l == csr_level) {
-    const int n = (int)ndof_level[l];
     const T* val = sizeof(T) == 8 ? (const T*)csr_v64 : (const T*)csr_v32;
-    hipLaunchKernelGGL(k_csr_spmv<T>, dim3(nblk(n)), dim3(BLK), 0, st, n, (const int32_t*)csr_rs,
+    hipLaunchKernelGGL(k_csr_spmv<T>, dim3(nblk(n), ncol), dim3(BLK), 0, st, n, (const int32_t*)csr_rs,
                        (const int32_t*)csr_col, val, x, y);
     return;
   }
@@ -321,59 +383,122 @@ void Pmg::apply_level(int l, const T* x, T* y, hipStream_t st) {
   op.level = l;
   tsem_space* s = l == 0 ? fine : sub[l - 1];
   T* yl = sizeof(T) == 8 ? (T*)s->yl64.p : (T*)s->yl32.p;
+  if (ncol > 1) {
+    if (sizeof(T) != 4 || ncol > bcap) throw std::logic_error("block V-cycle: FP32 columns within the reserve");
+    op.ncol = ncol;
+    op.ldx = n;
+    op.ldl = yl_cols(l);
+    yl = as<T>(byl[l]);
+  }
   launch_apply<T>(s->d, op, x, y, yl, nullptr, st);
 }
 
 // Chebyshev iteration of the given degree on D^-1 A_l x = D^-1 b over
 // [hi / 30, hi], hi = 1.1 lmax (the power iteration approaches lmax from below)
 template <typename T>
-void Pmg::smooth_level(int l, T* x, const T* b, int degree, bool zero_init, hipStream_t st) {
+void Pmg::smooth_level(int l, T* x, const T* b, int degree, bool zero_init, hipStream_t st, int ncol) {
   const int n = (int)ndof_level[l];
   const DevSpace& d = dev(l);
-  T* Ax = as<T>(vr[l]);
-  T* dv = as<T>(vd[l]);
+  T* Ax = as<T>(ncol > 1 ? br[l] : vr[l]);
+  T* dv = as<T>(ncol > 1 ? bd[l] : vd[l]);
+  const dim3 grid(nblk(n), ncol);
   const T* inv = (sizeof(T) == 8) ? (const T*)d.invdiag : (const T*)d.invdiag32;
   const double hi = 1.1 * lmax[l], lo = hi / 30.0;
   const double theta = (hi + lo) / 2, delta = (hi - lo) / 2, sigma = theta / delta;
   double rho_old = 1.0 / sigma;
-  if (!zero_init) apply_level<T>(l, x, Ax, st);
-  hipLaunchKernelGGL(k_cheb<T>, dim3(nblk(n)), dim3(BLK), 0, st, n, b, zero_init ? (const T*)nullptr : (const T*)Ax,
-                     inv, T(0), T(1.0 / theta), dv, x);
+  if (!zero_init) apply_level<T>(l, x, Ax, st, ncol);
+  hipLaunchKernelGGL(k_cheb<T>, grid, dim3(BLK), 0, st, n, b, zero_init ? (const T*)nullptr : (const T*)Ax, inv, T(0),
+                     T(1.0 / theta), dv, x);
   for (int k = 1; k < degree; ++k) {
-    apply_level<T>(l, x, Ax, st);
+    apply_level<T>(l, x, Ax, st, ncol);
     const double rho = 1.0 / (2 * sigma - rho_old);
-    hipLaunchKernelGGL(k_cheb<T>, dim3(nblk(n)), dim3(BLK), 0, st, n, b, (const T*)Ax, inv, T(rho * rho_old),
+    hipLaunchKernelGGL(k_cheb<T>, grid, dim3(BLK), 0, st, n, b, (const T*)Ax, inv, T(rho * rho_old),
                        T(2 * rho / delta), dv, x);
     rho_old = rho;
   }
 }
 
 template <typename T>
-void Pmg::vcycle(const T* r, T* z, int smooth, hipStream_t st) {
+void Pmg::vcycle(const T* r, T* z, int smooth, hipStream_t st, int ncol) {
   const int L = levels();
   if (smooth < 1) smooth = 1;
+  if (ncol < 1) throw std::invalid_argument("V-cycle needs at least one column");
+  if (ncol > 1) {
+    if (sizeof(T) != 4 || !block_supported()) throw std::logic_error("block V-cycle needs the FP32 block operator");
+    block_reserve(ncol);
+  }
   std::vector<T*> X(L), B(L);
   X[0] = z;
   B[0] = const_cast<T*>(r);
-  for (int l = 1; l < L; ++l) { X[l] = as<T>(vx[l]); B[l] = as<T>(vb[l]); }
-  for (int l = 0; l + 1 < L; ++l) {
-    smooth_level<T>(l, X[l], B[l], smooth, true, st);
-    T* Ax = as<T>(vr[l]);
-    apply_level<T>(l, X[l], Ax, st);
-    const int nc = (int)ndof_level[l + 1];
-    hipLaunchKernelGGL(k_restrict_res<T>, dim3(nblk(nc)), dim3(BLK), 0, st, nc, (const int32_t*)map[l + 1],
-                       (const T*)B[l], (const T*)Ax, B[l + 1]);
+  for (int l = 1; l < L; ++l) {
+    X[l] = as<T>(ncol > 1 ? bx[l] : vx[l]);
+    B[l] = as<T>(ncol > 1 ? bb[l] : vb[l]);
   }
-  smooth_level<T>(L - 1, X[L - 1], B[L - 1], L > 1 ? coarse_deg : smooth, true, st);
+  for (int l = 0; l + 1 < L; ++l) {
+    smooth_level<T>(l, X[l], B[l], smooth, true, st, ncol);
+    T* Ax = as<T>(ncol > 1 ? br[l] : vr[l]);
+    apply_level<T>(l, X[l], Ax, st, ncol);
+    const int nc = (int)ndof_level[l + 1];
+    hipLaunchKernelGGL(k_restrict_res<T>, dim3(nblk(nc), ncol), dim3(BLK), 0, st, nc, ndof_level[l],
+                       (const int32_t*)map[l + 1], (const T*)B[l], (const T*)Ax, B[l + 1]);
+  }
+  smooth_level<T>(L - 1, X[L - 1], B[L - 1], L > 1 ? coarse_deg : smooth, true, st, ncol);
   for (int l = L - 2; l >= 0; --l) {
     const int nc = (int)ndof_level[l + 1];
-    hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(nc)), dim3(BLK), 0, st, nc, (const int32_t*)map[l + 1],
-                       (const T*)X[l + 1], X[l]);
-    smooth_level<T>(l, X[l], B[l], smooth, false, st);
+    hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(nc), ncol), dim3(BLK), 0, st, nc, ndof_level[l],
+                       (const int32_t*)map[l + 1], (const T*)X[l + 1], X[l]);
+    smooth_level<T>(l, X[l], B[l], smooth, false, st, ncol);
   }
 }
 
-template void Pmg::vcycle<double>(const double*, double*, int, hipStream_t);
-template void Pmg::vcycle<float>(const float*, float*, int, hipStream_t);
+template void Pmg::vcycle<double>(const double*, double*, int, hipStream_t, int);
+template void Pmg::vcycle<float>(const float*, float*, int, hipStream_t, int);
+
+template <typename T>
+void Pmg::vcycle_replay(const T* r, T* z, int smooth, hipStream_t st, int ncol) {
+  const std::vector<const void*> key = {r, z, (const void*)(intptr_t)smooth, (const void*)sizeof(T),
+                                        (const void*)(intptr_t)ncol, (const void*)(intptr_t)apply_path(),
+                                        (const void*)(intptr_t)slab_enabled()};
+  if (!graphs_enabled() || timing_on() || key == vfailed) {
+    vcycle<T>(r, z, smooth, st, ncol);
+    return;
+  }
+  if (ncol > 1) block_reserve(ncol);  // no allocation inside the capture
+  if (!vexec || vkey != key) {
+    if (vexec) (void)hipGraphExecDestroy(vexec);
+    vexec = nullptr;
+    vkey.clear();
+    hipGraph_t g = nullptr;
+    bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+      try {
+        vcycle<T>(r, z, smooth, st, ncol);
+      } catch (...) {
+        ok = false;
+      }
+      if (hipStreamEndCapture(st, &g) != hipSuccess) ok = false;
+      if (ok && g && hipGraphInstantiate(&vexec, g, nullptr, nullptr, 0) != hipSuccess) {
+        vexec = nullptr;
+        ok = false;
+      }
+      if (g) (void)hipGraphDestroy(g);
+    }
+    if (!ok || !vexec) {
+      // the captured launches never ran: this V-cycle runs eagerly, and so
+      // does every later one with this key
+      if (vexec) (void)hipGraphExecDestroy(vexec);
+      vexec = nullptr;
+      vfailed = key;
+      (void)hipGetLastError();
+      vcycle<T>(r, z, smooth, st, ncol);
+      return;
+    }
+    vkey = key;
+  }
+  HIPCHK(hipGraphLaunch(vexec, st));
+}
+
+template void Pmg::vcycle_replay<double>(const double*, double*, int, hipStream_t, int);
+template void Pmg::vcycle_replay<float>(const float*, float*, int, hipStream_t, int);
 
 }  // namespace tsem

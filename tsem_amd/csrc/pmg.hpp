@@ -39,9 +39,20 @@ class Pmg {
   Pmg& operator=(const Pmg&) = delete;
   // z = V(r): one V-cycle from a zero initial guess; `smooth` is the
   // Chebyshev degree of the pre- and post-smoothing on every level but the
-  // coarsest (which gets coarse_deg)
+  // coarsest (which gets coarse_deg).  ncol > 1: V-cycles of ncol columns
+  // (column c of r and z at c N_0) with every level's operator, smoother
+  // and transfer one launch over the columns; FP32 only, and only when
+  // block_supported().  Each column gets the same arithmetic as alone.
   template <typename T>
-  void vcycle(const T* r, T* z, int smooth, hipStream_t st);
+  void vcycle(const T* r, T* z, int smooth, hipStream_t st, int ncol = 1);
+  // the same V-cycle captured once as a graph and replayed while r, z and
+  // the parameters stay the same (eager when graphs or kernel timing are
+  // on, or when the capture failed); for callers that run many V-cycles
+  // outside a captured solver chunk
+  template <typename T>
+  void vcycle_replay(const T* r, T* z, int smooth, hipStream_t st, int ncol = 1);
+  // every level's FP32 operator has a block form (fp16-split path)
+  bool block_supported() const;
   int levels() const { return (int)order.size(); }
   std::vector<int> order;        // polynomial order per level (level 0 = the fine space)
   std::vector<double> lmax;      // spectral radius estimate of D^-1 A_l
@@ -64,10 +75,18 @@ class Pmg {
   void assemble_level(int l, hipStream_t st);
   const DevSpace& dev(int l) const;
   void release();
+  hipGraphExec_t vexec = nullptr;
+  std::vector<const void*> vkey, vfailed;
+  // column blocks of the level vectors and element results (FP32, bcap
+  // columns; grown by block_reserve, never inside a capture)
+  int bcap = 0;
+  std::vector<void*> bx, bb, br, bd, byl;
+  void block_reserve(int ncol);
+  long yl_cols(int l) const;  // FP32 element-result entries of one column
   template <typename T>
-  void apply_level(int l, const T* x, T* y, hipStream_t st);
+  void apply_level(int l, const T* x, T* y, hipStream_t st, int ncol = 1);
   template <typename T>
-  void smooth_level(int l, T* x, const T* b, int degree, bool zero_init, hipStream_t st);
+  void smooth_level(int l, T* x, const T* b, int degree, bool zero_init, hipStream_t st, int ncol = 1);
 };
 
 }  // namespace tsem

@@ -90,6 +90,7 @@ SIGNATURES = {
     "tsem_serve_poll": [_vp, C.c_int64, C.c_int, _vp, _vp, _vp],
     "tsem_serve_close": [_vp],
     "tsem_set_graphs": [C.c_int],
+    "tsem_set_block_vcycle": [C.c_int],
     "tsem_timing": [C.c_int],
     "tsem_timing_read": [_dp],
     "tsem_pcg": [_vp, _vp, _vp, C.POINTER(PcgOpts), C.POINTER(PcgStats)],
