@@ -82,15 +82,26 @@ __global__ void __launch_bounds__(256) k_gram(int n, int p, int q, const double*
     }
 }
 
-// out[v] = sum_b partial[b nvals + v] in block order (neighbouring threads
-// read neighbouring values)
+// out[v] = sum_b partial[b nvals + v] in a fixed order: a block takes 32
+// values, lane group g (of 8) sums the partials b = g, g + 8, ... in
+// order, then the 8 group sums are added in group order.  Neighbouring
+// lanes read neighbouring values.
+constexpr int RV = 32, RG = 8;
 __global__ void __launch_bounds__(256) k_gram_reduce(int nvals, int nb, const double* __restrict__ partial,
                                                      double* __restrict__ out) {
-  int v = blockIdx.x * 256 + threadIdx.x;
-  if (v >= nvals) return;
+  __shared__ double red[RG][RV];
+  const int vl = threadIdx.x % RV, g = threadIdx.x / RV;
+  const int v = blockIdx.x * RV + vl;
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += partial[(size_t)b * nvals + v];
-  out[v] = s;
+  if (v < nvals)
+    for (int b = g; b < nb; b += RG) s += partial[(size_t)b * nvals + v];
+  red[g][vl] = s;
+  __syncthreads();
+  if (g == 0 && v < nvals) {
+    double t = red[0][vl];
+    for (int k = 1; k < RG; ++k) t += red[k][vl];
+    out[v] = t;
+  }
 }
 
 void launch_gram(int n, int p, int q, const double* U, long ldu, const double* V, long ldv, double* partial,
@@ -108,80 +119,145 @@ void launch_gram(int n, int p, int q, const double* U, long ldu, const double* V
   else
     hipLaunchKernelGGL(k_gram<6>, dim3(nb), dim3(256), 0, st, n, p, q, U, ldu, V, ldv, rpb, partial);
   int nv = p * q;
-  hipLaunchKernelGGL(k_gram_reduce, dim3((nv + 255) / 256), dim3(256), 0, st, nv, nb, partial, out);
+  hipLaunchKernelGGL(k_gram_reduce, dim3((nv + RV - 1) / RV), dim3(256), 0, st, nv, nb, partial, out);
 }
 
 // Y[r][j] = alpha sum_c S[r][c] C[c][j] + beta Y[r][j], C row-major p x m
 // (m <= 16 MT); Y is not read when beta == 0.  On the FP64 matrix cores as
-// Y^T = C^T S^T: per 16-row tile of S one wave holds the tile's p entries
-// (lane (lr, lk) the entries S[r0 + lr][4 k + lk], all loads in flight at
-// once), multiplies them by C^T from LDS with v_mfma_f64_16x16x4f64 and
-// writes rows r0 .. r0 + 15 of each output column (value q of lane (lr, lk)
-// is Y[r0 + lr][16 t + lk + 4 q]).  A wave reads its rows whole before it
-// writes them and waves own disjoint rows, so Y may be columns of S itself
-// (an in-place update).
-constexpr int CRT = 8;  // row tiles per wave
-template <int MT>
+// Y^T = C^T S^T, 64 rows of S at a time: the block stages the rows in LDS
+// (column-major, coalesced loads; the next rows are loaded into registers
+// meanwhile), wave w holds the C^T tile of output columns 16 (w % MT) .. + 15
+// in registers for the whole launch (lane (lr, lk): C[4 k + lk][16 t + lr])
+// and multiplies it into MT of the four 16-row tiles with
+// v_mfma_f64_16x16x4f64 (value q of lane (lr, lk) is Y[r0 + lr][16 t + lk +
+// 4 q]).  Output tiles from column jsplit on skip the first ksplit rows of C
+// (zero there).  Every row of S a block reads is in LDS before any of the
+// block's writes of that row, and blocks own disjoint rows, so Y may be
+// columns of S itself (an in-place update).
+constexpr int CS = 64;        // rows per stage
+constexpr int CSP = CS + 16;  // padded LDS row: the four 16-lane groups of a read hit other banks
+// KS: k steps of four columns of S (p <= 4 KS); BETA: Y is read
+template <int MT, int KS, bool BETA>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_combine(int n, int p, int m, const double* S, long lds,
                                                  const double* __restrict__ C, double* Y, long ldy, double alpha,
-                                                 double beta) {
+                                                 double beta, int jsplit, int ksplit, int stages_per_block) {
   typedef double v4 __attribute__((ext_vector_type(4)));
-  constexpr int MW = 16 * MT;
-  constexpr int KS = GP / 4;  // k steps of 4 columns of S
-  __shared__ double sc[GP * MW];  // C zero padded to 96 x MW
+  constexpr int WPT = 4 / MT;  // waves per output tile
+  constexpr int RTW = 4 / WPT; // 16-row tiles per wave and stage
+  __shared__ double sS[GP * CSP];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
+  const int t = wave % MT, rw = wave / MT;
   const int pp = (p + 3) & ~3;
-  for (int s = threadIdx.x; s < pp * MW; s += 256) {
-    const int c = s / MW, j = s % MW;
-    sc[s] = (c < p && j < m) ? C[(size_t)c * m + j] : 0.0;
+  const int kb = (16 * t >= jsplit) ? ksplit / 4 : 0;
+  double a[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int c = 4 * k + lk, j = 16 * t + lr;
+    a[k] = (c < p && j < m) ? C[(size_t)c * m + j] : 0.0;
   }
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lk = lane >> 4;
-  for (int rt = 0; rt < CRT; ++rt) {
-    const long r0 = ((long)blockIdx.x * 4 * CRT + (long)wave * CRT + rt) * 16;
-    if (r0 >= n) break;  // wave-uniform
-    const long r = r0 + lr;
-    const bool ok = r < n;
-    double sv[KS];
+  const long nst = ((long)n + CS - 1) / CS;
+  const long s0 = (long)blockIdx.x * stages_per_block;
+  const long s1 = s0 + stages_per_block < nst ? s0 + stages_per_block : nst;
+  // staging: row, first column (then every 4th).  CS = 64: the column is
+  // the wave index, uniform, so each load's column address is scalar
+  static_assert(CS == 64, "one staged column per wave and load");
+  const int lrow = lane, lcol = __builtin_amdgcn_readfirstlane(wave);
+  double nx[KS];
+  auto fetch = [&](long sg) {
+    const long r = sg * CS + lrow;
 #pragma unroll
-    for (int k = 0; k < KS; ++k)
-      if (4 * k < pp) sv[k] = (ok && 4 * k + lk < p) ? S[(size_t)(4 * k + lk) * lds + r] : 0.0;
-    v4 acc[MT];
+    for (int u = 0; u < KS; ++u) {
+      const int c = lcol + 4 * u;
+      nx[u] = (c < p && r < n) ? S[(size_t)c * lds + r] : 0.0;
+    }
+  };
+  if (s0 < s1) fetch(s0);
+  for (long sg = s0; sg < s1; ++sg) {
+    __syncthreads();  // the previous stage's LDS reads are done
 #pragma unroll
-    for (int t = 0; t < MT; ++t) acc[t] = v4{0, 0, 0, 0};
+    for (int u = 0; u < KS; ++u) {
+      const int c = lcol + 4 * u;
+      if (c < pp) sS[c * CSP + lrow] = nx[u];
+    }
+    __syncthreads();
+    if (sg + 1 < s1) fetch(sg + 1);  // in flight during the products
+    const long r0 = sg * CS;
+    double yv[RTW][4];
+    if (BETA) {
 #pragma unroll
-    for (int k = 0; k < KS; ++k)
-      if (4 * k < pp) {
-#pragma unroll
-        for (int t = 0; t < MT; ++t)
-          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(sc[(4 * k + lk) * MW + 16 * t + lr], sv[k], acc[t], 0, 0, 0);
-      }
-    if (ok) {
-#pragma unroll
-      for (int t = 0; t < MT; ++t)
+      for (int j = 0; j < RTW; ++j)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int j = 16 * t + lk + 4 * q;
-          if (j < m) {
-            double* y = Y + (size_t)j * ldy + r;
-            *y = (beta != 0.0) ? alpha * acc[t][q] + beta * *y : alpha * acc[t][q];
-          }
+          const long r = r0 + 16 * (rw * RTW + j) + lr;
+          const int jj = 16 * t + lk + 4 * q;
+          yv[j][q] = (jj < m && r < n) ? Y[(size_t)jj * ldy + r] : 0.0;
         }
     }
+    v4 acc[RTW];
+#pragma unroll
+    for (int j = 0; j < RTW; ++j) acc[j] = v4{0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      if (k >= kb && 4 * k < pp) {
+#pragma unroll
+        for (int j = 0; j < RTW; ++j) {
+          const double bv = sS[(4 * k + lk) * CSP + 16 * (rw * RTW + j) + lr];
+          acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[k], bv, acc[j], 0, 0, 0);
+        }
+      }
+      // keep the LDS reads of later k steps from being hoisted here (their
+      // registers would cut the waves per SIMD)
+      if (k % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int j = 0; j < RTW; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const long r = r0 + 16 * (rw * RTW + j) + lr;
+        const int jj = 16 * t + lk + 4 * q;
+        if (jj < m && r < n) {
+          const double v = alpha * acc[j][q];
+          Y[(size_t)jj * ldy + r] = BETA ? v + beta * yv[j][q] : v;
+        }
+      }
   }
+}
+
+static void combine_impl(int n, int p, int m, const double* S, long lds, const double* C, double* Y, long ldy,
+                         double alpha, double beta, int jsplit, int ksplit, hipStream_t st) {
+  if (p < 1 || m < 1) return;
+  if (p > GP || m > CM) throw std::invalid_argument("block combination limited to 96 x 64 coefficients");
+  const long nst = ((long)n + CS - 1) / CS;
+  const int nb = (int)(nst < 1024 ? nst : 1024);
+  const int spb = (int)((nst + nb - 1) / nb);
+  const bool b = beta != 0.0;
+  const int ks = p <= 32 ? 8 : p <= 64 ? 16 : 24;
+  const int mt = m <= 16 ? 1 : m <= 32 ? 2 : 4;
+#define TSEM_COMBINE(MT, KS, B)                                                                                   \
+  if (mt == MT && ks == KS && b == B) {                                                                           \
+    hipLaunchKernelGGL((k_combine<MT, KS, B>), dim3(nb), dim3(256), 0, st, n, p, m, S, lds, C, Y, ldy, alpha, beta, \
+                       jsplit, ksplit, spb);                                                                      \
+    return;                                                                                                       \
+  }
+  TSEM_COMBINE(1, 8, false) TSEM_COMBINE(2, 8, false) TSEM_COMBINE(4, 8, false)
+  TSEM_COMBINE(1, 16, false) TSEM_COMBINE(2, 16, false) TSEM_COMBINE(4, 16, false)
+  TSEM_COMBINE(1, 24, false) TSEM_COMBINE(2, 24, false) TSEM_COMBINE(4, 24, false)
+  TSEM_COMBINE(1, 8, true) TSEM_COMBINE(2, 8, true) TSEM_COMBINE(4, 8, true)
+  TSEM_COMBINE(1, 16, true) TSEM_COMBINE(2, 16, true) TSEM_COMBINE(4, 16, true)
+  TSEM_COMBINE(1, 24, true) TSEM_COMBINE(2, 24, true) TSEM_COMBINE(4, 24, true)
+#undef TSEM_COMBINE
+  throw std::logic_error("block combination: no kernel for this shape");
 }
 
 void launch_combine(int n, int p, int m, const double* S, long lds, const double* C, double* Y, long ldy,
                     double alpha, double beta, hipStream_t st) {
-  if (p < 1 || m < 1) return;
-  if (p > GP || m > CM) throw std::invalid_argument("block combination limited to 96 x 64 coefficients");
-  const long rows_per_block = 4L * CRT * 16;
-  const dim3 grid((unsigned)((n + rows_per_block - 1) / rows_per_block));
-  if (m <= 16)
-    hipLaunchKernelGGL(k_combine<1>, grid, dim3(256), 0, st, n, p, m, S, lds, C, Y, ldy, alpha, beta);
-  else if (m <= 32)
-    hipLaunchKernelGGL(k_combine<2>, grid, dim3(256), 0, st, n, p, m, S, lds, C, Y, ldy, alpha, beta);
-  else
-    hipLaunchKernelGGL(k_combine<4>, grid, dim3(256), 0, st, n, p, m, S, lds, C, Y, ldy, alpha, beta);
+  combine_impl(n, p, m, S, lds, C, Y, ldy, alpha, beta, m, 0, st);
+}
+
+void launch_combine_split(int n, int p, int m, const double* S, long lds, const double* C, double* Y, long ldy,
+                          int jsplit, int ksplit, hipStream_t st) {
+  if (jsplit % 16 || ksplit % 4 || ksplit > p) throw std::invalid_argument("combination split off its tiles");
+  combine_impl(n, p, m, S, lds, C, Y, ldy, 1.0, 0.0, jsplit, ksplit, st);
 }
 
 // Y[:, j] = (float)(X[:, j] s[j]) and W[:, j] = s[j] Z[:, j] for blocks of

@@ -427,7 +427,12 @@ int tsem_lobpcg(tsem_space* sp, int m, double* X, double* lambda, const tsem_lob
         if (p > m && i >= m) C2[(size_t)i * m2 + m + j] = C[(size_t)i * m + j];
       }
     const double* dc = upload_coef(C2);
-    for (int b = 0; b < 3; ++b) launch_combine(n, p, m2, base[b], n, dc, base[b], n, 1.0, 0.0, st);
+    for (int b = 0; b < 3; ++b) {
+      if (m2 > m && m % 16 == 0)
+        launch_combine_split(n, p, m2, base[b], n, dc, base[b], n, m, m, st);
+      else
+        launch_combine(n, p, m2, base[b], n, dc, base[b], n, 1.0, 0.0, st);
+    }
     np = 0;
     if (p > m) {
       gram_host(m, m, col(0, m), col(2, m), GB);

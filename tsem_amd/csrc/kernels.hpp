@@ -244,6 +244,10 @@ void launch_gram(int n, int p, int q, const double* U, long ldu, const double* V
 // Y = alpha S C + beta Y with C row-major p x m on device (p <= 96, m <= 32)
 void launch_combine(int n, int p, int m, const double* S, long lds, const double* C, double* Y, long ldy,
                     double alpha, double beta, hipStream_t st);
+// Y = S C where the output columns from jsplit on take only rows ksplit.. of
+// C (the rows above are zero); jsplit a multiple of 16, ksplit of 4
+void launch_combine_split(int n, int p, int m, const double* S, long lds, const double* C, double* Y, long ldy,
+                          int jsplit, int ksplit, hipStream_t st);
 // per-column scaled casts of blocks (ncol columns n apart, scales on the device)
 void launch_cast_cols(int n, int ncol, const double* X, const double* s, float* Y, hipStream_t st);
 void launch_widen_cols(int n, int ncol, const float* Z, const double* s, double* W, hipStream_t st);
