@@ -379,13 +379,17 @@ int tsem_set_potential(tsem_space* sp, const double* v_dev);
 int tsem_apply_potential(tsem_space* sp, double stiff, double mass, double potential, int constrained,
                          const double* x_dev, double* y_dev);
 
-/* ---- served stream of Poisson requests -------------------------------
+/* ---- served stream of Poisson and eigen requests ----------------------
  * The reference has no served form; these four entry points are the
- * boundary of a service that solves many independent Poisson problems on
- * one device.  A request is the mixed-BC problem of
+ * boundary of a service that solves many independent problems on one
+ * device.  A Poisson request is the mixed-BC problem of
  * examples/poisson_mixBC/poisson.cpp:74-89 (load vector, Neumann load on
  * mark 2, Dirichlet elimination on mark 1), solved by PCG to a relative
- * residual.  Workers (one context each) take the earliest request of their
+ * residual; an eigen request the lowest pairs of the Dirichlet Laplacian by
+ * LOBPCG; a system request a PCG solve of the caller's own assembled
+ * system (its space, b and x), e.g. the Hartree solve of a Kohn-Sham SCF
+ * step (examples/KohnSham-Helium/SCFIteration.cpp:154-177).  Workers (one
+ * context each) take the earliest request of their
  * lane -- the first small_slots workers only small requests, the others the
  * earliest large one, else the earliest small one -- as soon as its device
  * footprint fits the budget; spaces are cached per topology key and only
@@ -429,7 +433,30 @@ typedef struct {
   double tol;             /* relative residual ||b - Ax||_2 / ||b||_2 (0: 1e-10) */
   int mixed;              /* tsem_pcg_opts.mixed (inner_tol 1e-2) */
   int precond;            /* 1 Jacobi (0 means 1), 4 p-multigrid */
+  int kind;               /* TSEM_REQ_POISSON, TSEM_REQ_EIGEN or TSEM_REQ_SYSTEM */
+  /* Poisson: the manufactured problem (copied by submit) in place of the
+   * callbacks f, u and g, evaluated on the device; NULL: the callbacks */
+  const tsem_manufactured_def* manufactured;
+  /* eigen: the nev (1..32) lowest pairs of S x = lambda M x with zero
+   * Dirichlet data on the Dirichlet parts, LOBPCG to residual tol (0: 1e-8)
+   * from an N(0, 1) block of the given seed, p-multigrid preconditioned;
+   * the eigenvalues go to eigenvalues (host, nev, written by the worker),
+   * the eigenvectors (n_dof x nev, column-major) to poll's x */
+  int nev;
+  uint64_t seed;
+  double* eigenvalues;
+  /* system: PCG on the caller's space (its current elimination) for b
+   * (device) from the initial guess x (device, solution out), on that
+   * space's context; the mesh fields are unused and the space must not be
+   * used by the caller until the request has finished */
+  tsem_space* space;
+  const double* b_dev;
+  double* x_dev;
 } tsem_serve_request;
+
+#define TSEM_REQ_POISSON 0
+#define TSEM_REQ_EIGEN 1
+#define TSEM_REQ_SYSTEM 2
 
 typedef struct {
   int status;             /* TSEM_OK, TSEM_ENOCONV, or the request's error */
@@ -449,9 +476,10 @@ int tsem_serve_open(int device, const tsem_serve_opts* opts, tsem_server** out);
 int tsem_serve_submit(tsem_server* s, const tsem_serve_request* req, int64_t* id);
 /* *done = 0 while request id runs (wait != 0 blocks until it has finished).
  * Once finished: *done = 1, res filled (n_dof included) and the request's
- * status returned (its message in tsem_last_error); with x (host, n_dof
- * doubles) the solution is copied there and the request released, with x
- * NULL it is kept for a later poll. */
+ * status returned (its message in tsem_last_error); with x (host: n_dof
+ * doubles, n_dof x nev for an eigen request, unused for a system request)
+ * the solution is copied there and the request released, with x NULL it is
+ * kept for a later poll. */
 int tsem_serve_poll(tsem_server* s, int64_t id, int wait, double* x, tsem_serve_result* res, int* done);
 /* Solves every queued request, then frees the server (results not polled are dropped). */
 int tsem_serve_close(tsem_server* s);

@@ -160,7 +160,7 @@ def test_native_server_reports_callback_failure(ctx):
     _say("native serve error")
     srv = serve.NativeServer(slots=1)
     xyz, tet = mesh.kuhn_cube(3)
-    bad = srv.submit(xyz, tet, 3, Broken(1), labels=mesh.cube_plane_marks(xyz))
+    bad = srv.submit(xyz, tet, 3, Broken(1), labels=mesh.cube_plane_marks(xyz), callbacks=True)
     good = _submit(srv, 3, 3, 2)
     with pytest.raises(lib.TsemError) as e:
         srv.poll(bad)
@@ -168,3 +168,95 @@ def test_native_server_reports_callback_failure(ctx):
     done, x, r = srv.poll(good)
     srv.close()
     assert done and r["status"] == 0 and r["relres"] <= 1e-10
+
+
+def test_native_manufactured_matches_callbacks(ctx):
+    """A Poisson request whose problem the library evaluates on the device
+    (no callbacks) and the same request through the Python callbacks, which
+    call the same device evaluator: the same solution bit for bit."""
+    from tsem_amd import mesh, problems
+    _say("native serve manufactured vs callbacks")
+    srv = serve.NativeServer(slots=2)
+    a = _submit(srv, 4, 4, 31)
+    xyz, tet = mesh.kuhn_cube(4, jitter=0.1, seed=31)
+    b = srv.submit(xyz, tet, 4, problems.Manufactured(31), labels=mesh.cube_plane_marks(xyz), callbacks=True)
+    _, xa, ra = srv.poll(a)
+    _, xb, rb = srv.poll(b)
+    srv.close()
+    assert ra["relres"] <= 1e-10 and np.array_equal(xa, xb)
+
+
+def test_native_eigen_requests_in_the_mix(ctx):
+    """Eigen requests served among Poisson requests: the 6 lowest
+    eigenvalues of the Dirichlet Laplacian (n = 3, p = 4) within 1e-8 of the
+    dense generalized eigensolve of the oracle's pencil, and the same
+    request alone on a one-worker server gives the same bits."""
+    import scipy.linalg as la
+    import harness
+    from tsem_amd import mesh
+    _say("native serve eigen")
+    A, B, free, xyz, tet = harness.dirichlet_pencil(3, 4)
+    w = la.eigh(A, B, eigvals_only=True, subset_by_index=[0, 5])
+    lab = mesh.cube_plane_marks(xyz)
+    srv = serve.NativeServer(slots=4, small_cost=3e4)
+    e1 = srv.submit_eigen(xyz, tet, 4, 6, labels=lab, seed=3, qorder=7, tol=1e-9)
+    pois = [_submit(srv, 5, 4, 7), _submit(srv, 4, 5, 8), _submit(srv, 6, 3, 9)]
+    _, X1, r1 = srv.poll(e1)
+    for i in pois:
+        _, _, r = srv.poll(i)
+        assert r["relres"] <= 1e-10
+    srv.close()
+    lam = r1["eigenvalues"]
+    print("[result] served eigenvalues max rel err %.2e" % (np.abs(lam - w) / w).max(), flush=True)
+    assert r1["status"] == 0 and np.abs(lam - w).max() <= 1e-8 * w[-1]
+    assert X1.shape == (6, r1["n_dof"]) and np.abs(X1[:, ~free]).max() == 0
+    alone = serve.NativeServer(slots=1)
+    e2 = alone.submit_eigen(xyz, tet, 4, 6, labels=lab, seed=3, qorder=7, tol=1e-9)
+    _, X2, r2 = alone.poll(e2)
+    alone.close()
+    assert np.array_equal(r2["eigenvalues"], lam) and np.array_equal(X2, X1)
+
+
+def test_native_system_request_matches_direct_solve(ctx):
+    """A system request (PCG on the caller's space, as the Kohn-Sham Hartree
+    solve uses) served beside other requests equals the same PCG called
+    directly, bit for bit."""
+    import torch
+    from tsem_amd import api, mesh, problems
+    _say("native serve system")
+    xyz, tet = mesh.kuhn_cube(5, jitter=0.1, seed=41)
+    sp = api.Space(ctx, 4, xyz, tet, qorder=7)
+    b = api.poisson_setup(sp, problems.Manufactured(41))
+    x1, st1 = sp.pcg(b, tol=1e-10, criterion=1, precond=1, mixed=True, inner_tol=1e-2, check_every=10)
+    srv = serve.NativeServer(slots=3, small_cost=3e4)
+    other = [_submit(srv, 6, 4, 42), _submit(srv, 4, 6, 43)]
+    x2 = sp.zeros()
+    rid = srv.submit_system(sp, b, x2, tol=1e-10, mixed=1, precond=1)
+    done, none, r = srv.poll(rid)
+    for i in other:
+        srv.poll(i)
+    srv.close()
+    torch.cuda.synchronize()
+    assert done and none is None and r["status"] == 0 and r["iterations"] == st1["iterations"]
+    assert torch.equal(x1, x2)
+    sp.close()
+
+
+def test_kohn_sham_hartree_through_the_server(ctx):
+    """The helium SCF with every step's Hartree solve served as a system
+    request gives the energies of the direct solves (the same PCG, so the
+    same bits)."""
+    from tsem_amd import ks
+    _say("kohn-sham served hartree")
+    srv = serve.NativeServer(slots=2)
+    a = ks.KohnShamAtom(ctx, n=8, p=3, server=srv)
+    ra = a.run(steps=3)
+    srv.close()
+    b = ks.KohnShamAtom(ctx, n=8, p=3)
+    rb = b.run(steps=3)
+    ea = [h["energy"] for h in a.history]
+    eb = [h["energy"] for h in b.history]
+    print("[result] served %s direct %s" % (ea, eb), flush=True)
+    assert ea == eb
+    a.sp.close()
+    b.sp.close()

@@ -33,8 +33,11 @@ class KohnShamAtom:
     """One nucleus of charge Z at the origin of [-half, half]^3."""
 
     def __init__(self, ctx, n=24, p=5, Z=2.0, occ=(2.0, 0.0, 0.0, 0.0), half=10.0, beta=2.0, qorder=None,
-                 hartree=True, xc=True):
+                 hartree=True, xc=True, server=None):
+        """server: a serve.NativeServer that takes the Hartree solve of every
+        SCF step as a system request (None: solved here directly)."""
         import torch
+        self.server = server
         self.Z, self.half = float(Z), float(half)
         self.occ = np.asarray(occ, dtype=np.float64)
         self.nstates = len(self.occ)
@@ -78,9 +81,16 @@ class KohnShamAtom:
         mom = sp.integrate(self.rho, moments=True)
         pv, pe, pf = self.bc_pts
         sp.dirichlet(self.multipole(pv, mom), self.multipole(pe, mom), self.multipole(pf, mom), 1, b)
-        x, st = sp.pcg(b, self.vH, tol=tol, criterion=1, precond=1, mixed=True, check_every=20)
-        self.vH = x
-        self.vH_qp = sp.qp_values(x)
+        if self.server is not None:
+            # the served Poisson solve of the step: queued among the server's
+            # other requests, on this space
+            rid = self.server.submit_system(sp, b, self.vH, tol=tol, mixed=1, precond=1)
+            _, _, res = self.server.poll(rid)
+            st = dict(iterations=res["iterations"], relres=res["relres"], served=True)
+        else:
+            x, st = sp.pcg(b, self.vH, tol=tol, criterion=1, precond=1, mixed=True, check_every=10)
+            self.vH = x
+        self.vH_qp = sp.qp_values(self.vH)
         return st
 
     def potential(self):

@@ -50,14 +50,8 @@ class Manufactured:
             LG = LG - 4 * pi * pi * float(k @ k) * c / nt
         return S, gS, G, gG, LG
 
-    def _device(self, p, what, normal=None):
-        """u, f or du/dn at device points by the library's one-pass
-        evaluator (tsem_manufactured), on torch's current stream; None when p
-        is not a float64 device tensor."""
-        import torch
-        if not (isinstance(p, torch.Tensor) and p.is_cuda and p.dtype == torch.float64):
-            return None
-        import ctypes as C
+    def definition(self):
+        """The problem as the library's tsem_manufactured_def (cached)."""
         from . import lib
         d = getattr(self, "_def", None)
         if d is None:
@@ -69,6 +63,18 @@ class Manufactured:
                 d.phi[m] = float(self.phi[m])
             d.amp = float(self.amp)
             self._def = d
+        return d
+
+    def _device(self, p, what, normal=None):
+        """u, f or du/dn at device points by the library's one-pass
+        evaluator (tsem_manufactured), on torch's current stream; None when p
+        is not a float64 device tensor."""
+        import torch
+        if not (isinstance(p, torch.Tensor) and p.is_cuda and p.dtype == torch.float64):
+            return None
+        import ctypes as C
+        from . import lib
+        d = self.definition()
         pts = p.reshape(-1, 3).contiguous()
         out = torch.empty(pts.shape[0], dtype=torch.float64, device=p.device)
         if pts.shape[0] == 0:

@@ -286,12 +286,17 @@ class ServeOpts(C.Structure):
                 ("hbm_budget", C.c_int64), ("classes", C.c_int)]
 
 
+REQ_POISSON, REQ_EIGEN, REQ_SYSTEM = 0, 1, 2
+
+
 class ServeRequest(C.Structure):
     _fields_ = [("topology", C.c_int64), ("M", C.c_int), ("qorder", C.c_int), ("nv", C.c_int), ("nt", C.c_int),
                 ("xyz", C.c_void_p), ("tet", C.c_void_p), ("labels", C.c_void_p),
                 ("dirichlet", C.c_int32), ("neumann", C.c_int32),
                 ("f", _LOAD), ("u", _VALUE), ("g", _FLUX), ("user", C.c_void_p),
-                ("tol", C.c_double), ("mixed", C.c_int), ("precond", C.c_int)]
+                ("tol", C.c_double), ("mixed", C.c_int), ("precond", C.c_int),
+                ("kind", C.c_int), ("manufactured", C.c_void_p), ("nev", C.c_int), ("seed", C.c_uint64),
+                ("eigenvalues", C.c_void_p), ("space", C.c_void_p), ("b_dev", C.c_void_p), ("x_dev", C.c_void_p)]
 
 
 class ServeResult(C.Structure):
@@ -311,10 +316,13 @@ class _DeviceArray:
 class NativeServer:
     """The served stream of the C ABI (tsem_serve_open / submit / poll /
     close): worker threads, lanes, space pool and solve classes in the
-    library; problem data through callbacks.  `problem` objects give f, u and
-    du/dn as torch functions of points (problems.Manufactured): f runs on the
-    library's stream for the request, u and du/dn on the device and come back
-    to the host."""
+    library.  Poisson requests take a problems.Manufactured (evaluated by
+    the library on the device, no Python in the request's path) or any
+    `problem` with f, u and du/dn as torch functions of points, called back
+    from the workers (f on the library's stream for the request, u and du/dn
+    on the device with the values back to the host).  Eigen requests
+    (submit_eigen) and system requests on a caller's space (submit_system,
+    e.g. a Hartree solve) share the workers and the lanes."""
 
     def __init__(self, device=0, slots=8, small_slots=-1, small_cost=0.0, hbm_budget=0, classes=True):
         self.device = device
@@ -324,8 +332,27 @@ class NativeServer:
         self.h = h
         self.keep = {}  # request id -> the callbacks and arrays it uses
 
+    def _submit(self, r, keep):
+        rid = C.c_int64()
+        L.check(L.load().tsem_serve_submit(self.h, C.byref(r), C.byref(rid)))
+        self.keep[rid.value] = keep
+        return rid.value
+
     def submit(self, xyz, tet, M, problem, labels=None, dirichlet=0b000001, neumann=0b111110, topology=0,
-               qorder=0, tol=1e-10, mixed=1, precond=1):
+               qorder=0, tol=1e-10, mixed=1, precond=1, callbacks=False):
+        """A Poisson request; a problem with a device definition
+        (problems.Manufactured) is evaluated by the library unless
+        callbacks=True."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        tet = np.ascontiguousarray(tet, dtype=np.int32)
+        lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.int32)
+        if hasattr(problem, "definition") and not callbacks:
+            d = problem.definition()
+            r = ServeRequest(topology, M, qorder, len(xyz), len(tet), xyz.ctypes.data, tet.ctypes.data,
+                             None if lab is None else lab.ctypes.data, dirichlet, neumann, _LOAD(), _VALUE(),
+                             _FLUX(), None, tol, int(mixed), precond, REQ_POISSON, C.addressof(d), 0, 0, None,
+                             None, None, None)
+            return self._submit(r, dict(cols=1, keep=(d,)))
         import torch
         dev = "cuda:%d" % self.device
 
@@ -355,32 +382,63 @@ class NativeServer:
             except Exception:
                 return 1
 
-        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
-        tet = np.ascontiguousarray(tet, dtype=np.int32)
-        lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.int32)
         cb = (_LOAD(load), _VALUE(value), _FLUX(flux))
         r = ServeRequest(topology, M, qorder, len(xyz), len(tet), xyz.ctypes.data, tet.ctypes.data,
                          None if lab is None else lab.ctypes.data, dirichlet, neumann, cb[0], cb[1], cb[2], None,
-                         tol, int(mixed), precond)
-        rid = C.c_int64()
-        L.check(L.load().tsem_serve_submit(self.h, C.byref(r), C.byref(rid)))
-        self.keep[rid.value] = cb
-        return rid.value
+                         tol, int(mixed), precond, REQ_POISSON, None, 0, 0, None, None, None, None)
+        return self._submit(r, dict(cols=1, keep=cb))
+
+    def submit_eigen(self, xyz, tet, M, nev, labels=None, dirichlet=0b111111, topology=0, qorder=0, tol=1e-8,
+                     seed=7):
+        """An eigen request: the nev lowest pairs of the Laplacian with zero
+        Dirichlet data on the labelled parts in `dirichlet`; poll returns the
+        eigenvectors (n_dof x nev, column-major flattened) and the result's
+        "eigenvalues"."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        tet = np.ascontiguousarray(tet, dtype=np.int32)
+        lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.int32)
+        lam = np.zeros(nev)
+        r = ServeRequest(topology, M, qorder, len(xyz), len(tet), xyz.ctypes.data, tet.ctypes.data,
+                         None if lab is None else lab.ctypes.data, dirichlet, 0, _LOAD(), _VALUE(), _FLUX(), None,
+                         tol, 1, 4, REQ_EIGEN, None, nev, seed, lam.ctypes.data, None, None, None)
+        return self._submit(r, dict(cols=nev, eig=lam))
+
+    def submit_system(self, space, b, x, tol=1e-10, mixed=1, precond=1):
+        """A system request: PCG on `space` (an api.Space, its current
+        elimination) for the device vector b into x (initial guess in,
+        solution out); both stay untouched by the caller until poll says
+        done.  Device work queued before the call on any stream is finished
+        first."""
+        import torch
+        torch.cuda.synchronize(space.device)
+        r = ServeRequest(0, 0, 0, 0, 0, None, None, None, 0, 0, _LOAD(), _VALUE(), _FLUX(), None, tol, int(mixed),
+                         precond, REQ_SYSTEM, None, 0, 0, None, space.h.value, b.data_ptr(), x.data_ptr())
+        return self._submit(r, dict(cols=0, keep=(b, x)))
 
     def poll(self, rid, wait=True):
-        """(done, x, result dict): x the solution (host) once done, and the
-        request released; (False, None, None) while it runs."""
+        """(done, x, result dict): x the solution (host) once done -- n_dof
+        values, n_dof x nev (row j = eigenvector j) for an eigen request,
+        None for a system request (its solution is in the caller's x) -- and
+        the request released; (False, None, None) while it runs."""
         res = ServeResult()
         done = C.c_int()
         rc = L.load().tsem_serve_poll(self.h, rid, 1 if wait else 0, None, C.byref(res), C.byref(done))
         if not done.value:
             L.check(rc)
             return False, None, None
-        x = np.zeros(res.n_dof)
+        info = self.keep.get(rid, dict(cols=1))
+        count = res.n_dof * info["cols"]
+        x = np.zeros(max(1, count))
         rc = L.load().tsem_serve_poll(self.h, rid, 1, x.ctypes.data, C.byref(res), C.byref(done))
         self.keep.pop(rid, None)
         L.check(rc)
-        return True, x, {k: getattr(res, k) for k, _ in ServeResult._fields_}
+        out = {k: getattr(res, k) for k, _ in ServeResult._fields_}
+        if info["cols"] == 0:
+            return True, None, out
+        if "eig" in info:
+            out["eigenvalues"] = info["eig"].copy()
+            return True, x[:count].reshape(info["cols"], res.n_dof), out
+        return True, x[:count], out
 
     def run(self, requests):
         """Serve Requests (n, p, seed, arrival; the Python Server's stream) on
