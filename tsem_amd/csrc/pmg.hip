@@ -103,6 +103,25 @@ __global__ void k_csr_spmv(int n, const int32_t* __restrict__ rs, const int32_t*
   y[o + i] = acc;
 }
 
+// one Chebyshev step on the assembled level, with its product fused in:
+// d = c1 d + c2 Dinv (b - A xin), xout = xin + d (xin and xout distinct: the
+// rows read their neighbours' xin)
+template <typename T>
+__global__ void k_cheb_csr(int n, const int32_t* __restrict__ rs, const int32_t* __restrict__ col,
+                           const T* __restrict__ val, const T* __restrict__ b, const T* __restrict__ xin,
+                           const T* __restrict__ inv, T c1, T c2, T* __restrict__ dv, T* __restrict__ xout) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  const size_t o = (size_t)blockIdx.y * n;
+  T acc = T(0);
+  for (int k = rs[i]; k < rs[i + 1]; ++k) acc += val[k] * xin[o + col[k]];
+  const T r = b[o + i] - acc;
+  T dn = c2 * inv[i] * r;
+  if (c1 != T(0)) dn += c1 * dv[o + i];
+  dv[o + i] = dn;
+  xout[o + i] = xin[o + i] + dn;
+}
+
 template <typename T>
 T* as(void* p) { return static_cast<T*>(p); }
 }  // namespace
@@ -406,6 +425,24 @@ void Pmg::smooth_level(int l, T* x, const T* b, int degree, bool zero_init, hipS
   const double hi = 1.1 * lmax[l], lo = hi / 30.0;
   const double theta = (hi + lo) / 2, delta = (hi - lo) / 2, sigma = theta / delta;
   double rho_old = 1.0 / sigma;
+  if (l == csr_level && zero_init) {
+    // the assembled coarsest level: each later step is one launch, its
+    // product fused in, alternating between x and the product buffer (free
+    // here); the first step writes the buffer that the last step leaves in x
+    T* buf[2] = {x, Ax};
+    int cur = (degree - 1) % 2;
+    hipLaunchKernelGGL(k_cheb<T>, grid, dim3(BLK), 0, st, n, b, (const T*)nullptr, inv, T(0), T(1.0 / theta), dv,
+                       buf[cur]);
+    const T* val = sizeof(T) == 8 ? (const T*)csr_v64 : (const T*)csr_v32;
+    for (int k = 1; k < degree; ++k) {
+      const double rho = 1.0 / (2 * sigma - rho_old);
+      hipLaunchKernelGGL(k_cheb_csr<T>, grid, dim3(BLK), 0, st, n, (const int32_t*)csr_rs, (const int32_t*)csr_col,
+                         val, b, (const T*)buf[cur], inv, T(rho * rho_old), T(2 * rho / delta), dv, buf[1 - cur]);
+      cur = 1 - cur;
+      rho_old = rho;
+    }
+    return;
+  }
   if (!zero_init) apply_level<T>(l, x, Ax, st, ncol);
   hipLaunchKernelGGL(k_cheb<T>, grid, dim3(BLK), 0, st, n, b, zero_init ? (const T*)nullptr : (const T*)Ax, inv, T(0),
                      T(1.0 / theta), dv, x);

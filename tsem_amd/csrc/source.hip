@@ -116,6 +116,12 @@ struct Cx {
   double re, im;
 };
 __device__ inline Cx mul(Cx a, Cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+// e[k] for a wave number k in 0..3 by selects: an array indexed at run time
+// would live in scratch memory
+__device__ inline Cx pick(const Cx (&e)[4], int k) {
+  const Cx a = (k & 1) ? e[1] : e[0], b = (k & 1) ? e[3] : e[2];
+  return (k & 2) ? b : a;
+}
 
 struct Phases {
   double c[16], s[16];  // cos and sin of the phases
@@ -131,7 +137,7 @@ __global__ void __launch_bounds__(BLK) k_manufactured(tsem_manufactured_def P, P
   Cx e[3][4];  // e[d][k] = e^{2 pi i k x_d}, k = 0..3
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    sincos(pi * pts[3 * p + d], &s[d], &c[d]);
+    sincospi(pts[3 * p + d], &s[d], &c[d]);
     const Cx e1 = {c[d] * c[d] - s[d] * s[d], 2 * s[d] * c[d]};
     e[d][0] = {1.0, 0.0};
     e[d][1] = e1;
@@ -144,7 +150,7 @@ __global__ void __launch_bounds__(BLK) k_manufactured(tsem_manufactured_def P, P
   const double inv_n = 1.0 / P.nterms;
   for (int m = 0; m < P.nterms; ++m) {
     const int k0 = (int)P.k[m][0], k1 = (int)P.k[m][1], k2 = (int)P.k[m][2];
-    const Cx z = mul(mul(Cx{ph.c[m], ph.s[m]}, e[0][k0]), mul(e[1][k1], e[2][k2]));  // e^{i (2 pi k.x + phi)}
+    const Cx z = mul(mul(Cx{ph.c[m], ph.s[m]}, pick(e[0], k0)), mul(pick(e[1], k1), pick(e[2], k2)));  // e^{i (2 pi k.x + phi)}
     G += z.re * inv_n;
     gG[0] -= 2 * pi * P.k[m][0] * z.im * inv_n;
     gG[1] -= 2 * pi * P.k[m][1] * z.im * inv_n;
