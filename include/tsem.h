@@ -389,9 +389,10 @@ int tsem_apply_potential(tsem_space* sp, double stiff, double mass, double poten
  * LOBPCG; a system request a PCG solve of the caller's own assembled
  * system (its space, b and x), e.g. the Hartree solve of a Kohn-Sham SCF
  * step (examples/KohnSham-Helium/SCFIteration.cpp:154-177).  Workers (one
- * context each) take the earliest request of their
- * lane -- the first small_slots workers only small requests, the others the
- * earliest large one, else the earliest small one -- as soon as its device
+ * context each) take the smallest waiting request of their lane by estimated
+ * work (elements x modes) -- the first small_slots workers only small
+ * requests, the others the smallest large one, else the smallest small one
+ * -- as soon as its device
  * footprint fits the budget; spaces are cached per topology key and only
  * their coordinates replaced; with classes, small requests run foreground
  * and large ones background (tsem_ctx_set_class). */

@@ -210,6 +210,7 @@ Pmg::Pmg(tsem_space* fsp, hipStream_t st) : fine(fsp) {
     if (p > 1) order.push_back(1);
   }
   if (const char* env = getenv("TSEM_PMG_COARSE")) coarse_deg = std::max(1, atoi(env));
+  if (const char* env = getenv("TSEM_PMG_CRATIO")) coarse_ratio = std::max(2.0, atof(env));
   const int L = (int)order.size();
   ndof_level.assign(L, 0);
   ndof_level[0] = S.ndof;
@@ -422,7 +423,7 @@ void Pmg::smooth_level(int l, T* x, const T* b, int degree, bool zero_init, hipS
   T* dv = as<T>(ncol > 1 ? bd[l] : vd[l]);
   const dim3 grid(nblk(n), ncol);
   const T* inv = (sizeof(T) == 8) ? (const T*)d.invdiag : (const T*)d.invdiag32;
-  const double hi = 1.1 * lmax[l], lo = hi / 30.0;
+  const double hi = 1.1 * lmax[l], lo = hi / (l + 1 == levels() && l > 0 ? coarse_ratio : 30.0);
   const double theta = (hi + lo) / 2, delta = (hi - lo) / 2, sigma = theta / delta;
   double rho_old = 1.0 / sigma;
   if (l == csr_level && zero_init) {

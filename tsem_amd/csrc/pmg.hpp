@@ -58,6 +58,7 @@ class Pmg {
   std::vector<double> lmax;      // spectral radius estimate of D^-1 A_l
   std::vector<long> ndof_level;  // N_l
   int coarse_deg = 20;
+  double coarse_ratio = 30.0;  // the coarsest level's Chebyshev interval [hi / ratio, hi]
 
  private:
   tsem_space* fine;

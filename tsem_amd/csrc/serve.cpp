@@ -1,9 +1,10 @@
 // Served stream of Poisson, eigen and system requests behind the C ABI
 // (tsem_serve_*).
 //
-// A server owns `slots` worker threads.  Each worker takes the earliest
-// submitted request of its lane (small-request workers take only small
-// requests, the others the earliest large one, else the earliest small one)
+// A server owns `slots` worker threads.  Each worker takes the smallest
+// submitted request of its lane by estimated work (small-request workers take
+// only small requests, the others the smallest large one, else the smallest
+// small one)
 // as soon as the request's device footprint fits the budget, and solves it
 // on a space of its own context (stream), so requests in flight overlap on
 // the device.  Spaces are cached per caller topology key: a later request
@@ -202,15 +203,18 @@ struct tsem_server {
     std::unique_lock<std::mutex> lk(mu);
     const bool small_lane = w < o.small_slots;
     for (;;) {
-      // small lane: the earliest small request; general lane: the earliest
-      // large one, else the earliest small one
+      // small lane: the smallest small request; general lane: the smallest
+      // large one, else the smallest small one (shortest job first by the
+      // work estimate; submission order among equals)
       Job* pick = nullptr;
-      if (!small_lane)
+      auto smallest = [&](bool want_small) {
+        Job* best = nullptr;
         for (Job* j : pending)
-          if (!small(*j)) { pick = j; break; }
-      if (!pick)
-        for (Job* j : pending)
-          if (small(*j)) { pick = j; break; }
+          if (small(*j) == want_small && (!best || cost(*j) < cost(*best))) best = j;
+        return best;
+      };
+      if (!small_lane) pick = smallest(false);
+      if (!pick) pick = smallest(true);
       if (pick) {
         // a system request runs on the caller's space: no pool entry
         Entry* e = pick->req.kind == TSEM_REQ_SYSTEM ? nullptr : reserve(*pick);

@@ -92,8 +92,9 @@ class Server:
         """slots workers; the first small_slots of them (default half) take only
         requests whose cost is at most small_cost (default: the median over the
         request distribution), so small requests never queue behind large
-        ones; the other workers take the earliest large request, else the
-        earliest small one."""
+        ones; the other workers take the smallest large request, else the
+        smallest small one (shortest job first: a queue of requests of mixed
+        sizes waits least in total that way)."""
         import torch
         self.device = device
         self.slots = slots
@@ -143,8 +144,8 @@ class Server:
 
     # ------------------------------------------------------------ workers --
     def _next(self, w):
-        """The next request for worker w (arrival order within its lane) and
-        its pool entry, or None when no request is left."""
+        """The next request for worker w (the smallest arrived one of its
+        lane) and its pool entry, or None when no request is left."""
         small_lane = w < self.small_slots
         with self.lock:
             while True:
@@ -154,6 +155,9 @@ class Server:
                 arrived = [r for r in self.pending if r.arrival <= now]
                 small = [r for r in arrived if cost(r.n, r.p) <= self.small_cost]
                 large = [r for r in arrived if cost(r.n, r.p) > self.small_cost]
+                # shortest first within a lane (estimated work), arrival order among equals
+                small.sort(key=lambda r: (cost(r.n, r.p), r.arrival))
+                large.sort(key=lambda r: (cost(r.n, r.p), r.arrival))
                 pick = (small[:1] if small_lane else (large[:1] or small[:1]))
                 if pick:
                     ent = self._reserve((pick[0].n, pick[0].p))
@@ -477,3 +481,128 @@ class NativeServer:
             L.check(L.load().tsem_serve_close(self.h))
             self.h = None
             self.keep.clear()
+
+
+def stream(count=2000, load=0.8, slots=8, seed=2026, sat_count=300, log=print):
+    """The served stream of the C ABI server at one offered load, measured
+    end to end: count Poisson requests (n uniform in 8..32, p in 3..8, seed =
+    request id, jittered Kuhn meshes, manufactured problems evaluated in the
+    library) with Poisson arrivals at `load` x the saturated rate.
+
+    Steps: (1) meshes made up front; (2) one request of every (n, p) bucket
+    served, so the pool holds each topology's space (server warm-up); (3)
+    saturated rate: the first sat_count requests submitted at once; (4) the
+    stream; (5) isolated time per bucket: one request of the bucket alone on
+    the idle server, its space warm (the second of two runs).  Latency =
+    completion - arrival; each request's latency is set against its bucket's
+    isolated time.  Returns a summary dict."""
+    import json
+    from . import mesh as M, problems as P
+    reqs = make_requests(count, seed=seed)
+    t = time.perf_counter()
+    data = {}
+    for i, r in enumerate(reqs):
+        xyz, tet = M.kuhn_cube(r.n, jitter=0.1, seed=r.seed)
+        data[r.id] = (xyz, tet, M.cube_plane_marks(xyz))
+        if (i + 1) % 500 == 0:
+            log("stream: %d meshes (%.0f s)" % (i + 1, time.perf_counter() - t))
+    srv = NativeServer(slots=slots, small_cost=float(np.median([6 * n ** 3 * comb(p + 3, 3) for n in range(8, 33)
+                                                                  for p in range(3, 9)])))
+
+    def submit(r):
+        xyz, tet, lab = data[r.id]
+        return srv.submit(xyz, tet, r.p, P.Manufactured(r.seed), labels=lab, topology=1000 * r.n + r.p)
+
+    first = {}
+    for r in reqs:
+        first.setdefault((r.n, r.p), r)
+    t = time.perf_counter()
+    ids = [submit(r) for r in first.values()]
+    for k, rid in enumerate(ids):
+        srv.poll(rid)
+        if (k + 1) % 25 == 0:
+            log("stream: warm-up %d of %d buckets (%.0f s)" % (k + 1, len(ids), time.perf_counter() - t))
+    # saturated rate
+    t = time.perf_counter()
+    ids = [submit(r) for r in reqs[:sat_count]]
+    for rid in ids:
+        srv.poll(rid)
+    sat_wall = time.perf_counter() - t
+    sat_rate = len(ids) / sat_wall
+    rate = load * sat_rate
+    log("stream: saturated %.2f requests/s over %d requests; offered %.2f requests/s" % (sat_rate, len(ids), rate))
+    arrivals = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, count))
+    sub = [None] * count
+    lock = threading.Lock()
+
+    def feeder():
+        t0 = feeder.t0
+        for i, r in enumerate(reqs):
+            dt = arrivals[i] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            rid = submit(r)
+            with lock:
+                sub[i] = (time.perf_counter() - t0, rid)
+
+    feeder.t0 = time.perf_counter()
+    th = threading.Thread(target=feeder)
+    th.start()
+    res = [None] * count
+    for i in range(count):
+        while True:
+            with lock:
+                s = sub[i]
+            if s is not None:
+                break
+            time.sleep(0.002)
+        _, _, out = srv.poll(s[1])
+        res[i] = out
+        if (i + 1) % 100 == 0:
+            log("stream: %d of %d done (%.0f s)" % (i + 1, count, time.perf_counter() - feeder.t0))
+    th.join()
+    # isolated time per bucket, on the idle server with the space warm
+    iso = {}
+    for key, r in first.items():
+        for _ in range(2):
+            _, _, out = srv.poll(submit(r))
+        iso[key] = out["total_s"] - out["queued_s"]
+    srv.close()
+    lat = np.array([sub[i][0] + res[i]["total_s"] - arrivals[i] for i in range(count)])
+    ratio = np.array([lat[i] / iso[(r.n, r.p)] for i, r in enumerate(reqs)])
+    dofs = sum(res[i]["n_dof"] for i in range(count))
+    wall = max(sub[i][0] + res[i]["total_s"] for i in range(count))
+    worst = [dict(n=reqs[i].n, p=reqs[i].p, latency_s=float(lat[i]), isolated_s=iso[(reqs[i].n, reqs[i].p)],
+                  queued_s=res[i]["queued_s"]) for i in np.argsort(ratio)[::-1][:5]]
+    small = np.array([iso[(r.n, r.p)] <= np.median(list(iso.values())) for r in reqs])
+    out = dict(workload="served_poisson_n8-32_p3-8", requests=count, slots=slots, load=load,
+               saturated_requests_per_s=sat_rate, offered_requests_per_s=rate, wall_s=wall, dofs_per_s=dofs / wall,
+               p50_ms=float(np.percentile(lat, 50)) * 1e3, p99_ms=float(np.percentile(lat, 99)) * 1e3,
+               latency_over_isolated={"p50": float(np.percentile(ratio, 50)), "p90": float(np.percentile(ratio, 90)),
+                                      "p99": float(np.percentile(ratio, 99)), "max": float(ratio.max()),
+                                      "p99_small_half": float(np.percentile(ratio[small], 99)),
+                                      "p99_large_half": float(np.percentile(ratio[~small], 99))},
+               isolated_ms={"p50": float(np.percentile(list(iso.values()), 50)) * 1e3,
+                            "max": float(max(iso.values())) * 1e3, "buckets": len(iso)},
+               max_relres=float(max(res[i]["relres"] for i in range(count))), worst=worst)
+    log("stream: " + json.dumps(out))
+    return out
+
+
+def main():
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="served Poisson stream on the C ABI server at one offered load")
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--load", type=float, default=0.8)
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--sat", type=int, default=300, help="requests of the saturated-rate measurement")
+    a = ap.parse_args()
+    out = stream(a.requests, a.load, a.slots, sat_count=a.sat,
+                 log=lambda s: print("[serve]", s, file=sys.stderr, flush=True))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
