@@ -356,15 +356,17 @@ def main():
     # the whole solve against HBM, by the algorithmic bytes of one Jacobi PCG
     # iteration (s bytes per value): the operator reads p and writes A p
     # (2 N s), reads the geometric factors (6 E s) and one int32 gather slot
-    # per local coefficient (4 E K) plus 12 B per extra face-combination
-    # term of the gather map (F_x = map terms - E K); the fused CG updates
-    # move 11 N s (x += alpha p: 3, r += alpha A p with z = D^-1 r and the
-    # fused r.z: 5, p = z + beta p: 3).  Intermediates this design writes and
-    # reads back (the fp16 split tile, the entity slab) are not algorithmic
-    # bytes; they are reported beside it.
+    # per local coefficient (4 E K), plus 12 B per face-combination term
+    # (F_x); the fused CG updates move 11 N s (x += alpha p: 3, r += alpha A p
+    # with z = D^-1 r and the fused r.z: 5, p = z + beta p: 3).  F_x counts
+    # the terms of face modes that combine several DOFs: faces in a diagonal
+    # variant (identity or reflection, every face of the Kuhn meshes) have
+    # none, and the kernels read one value per such mode, so F_x = 0 here
+    # (with it the 444 MB per FP32 iteration of the 32^3, p = 6 mesh).
+    # Intermediates this design writes and reads back (the entity slab) are
+    # not algorithmic bytes; they are reported beside it.
     def b_iter(sz):
-        fx = max(0.0, nnz_sc - E * K)
-        return 2.0 * N * sz + 6.0 * E * sz + 4.0 * E * K + 12.0 * fx + 11.0 * N * sz
+        return 2.0 * N * sz + 6.0 * E * sz + 4.0 * E * K + 11.0 * N * sz
 
     it_bytes = b_iter(4.0)
     # the element results (entity slab or element-major), written and read back
