@@ -352,6 +352,265 @@ __global__ void __launch_bounds__(NT) k_elem_gemm(DevSpace S, OpCoef op, const T
   }
 }
 
+// The stiffness operator with the gather fused in: the block's element
+// coefficients go from x straight into an LDS tile X[k][element] (the layout
+// the tiled gather writes to HBM, here never leaving the CU), and GEMM 1 reads
+// its B operand from there.  X and the staging tables share the LDS of the V
+// tile: each wave keeps its V values in registers until every wave is done
+// with X, then writes them.  The gather is that of k_gather_tiles (descriptors,
+// masks, face-table rows and slot codes staged in LDS, all staging loads
+// issued before the first wait); the GEMMs and the Y write are those of
+// k_elem_gemm.
+template <typename T, int CT>
+__global__ void __launch_bounds__(NT, 3) k_elem_gemm_gx(DevSpace S, OpCoef op, const T* __restrict__ Dp,
+                                                     const T* __restrict__ x, T* __restrict__ yl) {
+  using v4 = typename Mma<T>::v4;
+  constexpr int NE = 16 * CT;
+  constexpr int VS = NE + 1;
+  constexpr int TPW = 2;  // m tiles per wave: Mt <= 8 (Mp <= 128)
+  extern __shared__ unsigned char smem_raw[];
+  T* V = reinterpret_cast<T*>(smem_raw);  // [3 Mp][VS], after GEMM 1
+  const int K = S.K;
+  const int Kl = op.Kl > 0 ? op.Kl : K;
+  const int Kdl = op.Kdl > 0 ? op.Kdl : S.Kd;
+  const int Mp = S.Mp, Cp = S.Cp, Kp = S.Kp;
+  const int Mt = (Kdl + 15) >> 4;
+  const int Rt = (Kl + 15) >> 4;
+  const int Kpl = (Kl + 3) & ~3;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lr = lane & 15, lk = lane >> 4;
+  const int e0 = blockIdx.x * NE;
+  const int nvalid = min(NE, S.nt - e0);
+  const T cst = (T)op.stiff;
+  // ---- phase 0: staging and the gather into X
+  const int R = 6 * S.ndg2, J = S.ftc_J;
+  T* X = reinterpret_cast<T*>(smem_raw);  // [Kp][VS]
+  ElemDesc* sd = reinterpret_cast<ElemDesc*>(X + (size_t)Kp * VS);
+  int* sm = reinterpret_cast<int*>(sd + NE);
+  T* cw = reinterpret_cast<T*>(sm + NE + (NE & 1));  // 8-byte aligned
+  int* cn = reinterpret_cast<int*>(cw + (size_t)R * J);
+  int* ck = cn + R;
+  int* sslot = ck + (size_t)R * J;
+  uint32_t* ssgn = reinterpret_cast<uint32_t*>(sslot + K);
+  const int dg = S.sl_diag;
+  {
+    constexpr int W = sizeof(ElemDesc) / 4;
+    constexpr int DQ = (NE * W + NT - 1) / NT;
+    constexpr int TQ = 2;
+    const int32_t* src = reinterpret_cast<const int32_t*>(S.desc + e0);
+    int32_t dv[DQ];
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+      const int s2 = tid + NT * q;
+      dv[q] = (s2 < NE * W && s2 / W < nvalid) ? src[s2] : 0;
+    }
+    const int gmv = (op.constrained && tid < nvalid) ? S.gmask[e0 + tid] : 0;
+    T cwv[TQ];
+    int ckv[TQ];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) {
+      const int s2 = tid + NT * q;
+      cwv[q] = s2 < R * J ? ((sizeof(T) == 8) ? (T)S.ftc_w[s2] : (T)S.ftc_w32[s2]) : T(0);
+      ckv[q] = s2 < R * J ? S.ftc_key[s2] : 0;
+    }
+    const int cnv = tid < R ? S.ftc_n[tid] : 0;
+    const int slv = tid < K ? S.slot[tid] : 0;
+    const uint32_t sgv = tid < 6 ? S.sl_sgn[tid] : 0u;
+    int32_t* dst = reinterpret_cast<int32_t*>(sd);
+#pragma unroll
+    for (int q = 0; q < DQ; ++q) {
+      const int s2 = tid + NT * q;
+      if (s2 < NE * W) dst[s2] = dv[q];
+    }
+    if (tid < NE) sm[tid] = gmv;
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) {
+      const int s2 = tid + NT * q;
+      if (s2 < R * J) { cw[s2] = cwv[q]; ck[s2] = ckv[q]; }
+    }
+    for (int s2 = tid + NT * TQ; s2 < R * J; s2 += NT) {
+      cw[s2] = (sizeof(T) == 8) ? (T)S.ftc_w[s2] : (T)S.ftc_w32[s2];
+      ck[s2] = S.ftc_key[s2];
+    }
+    if (tid < R) cn[tid] = cnv;
+    for (int s2 = tid + NT; s2 < R; s2 += NT) cn[s2] = S.ftc_n[s2];
+    if (tid < K) sslot[tid] = slv;
+    for (int s2 = tid + NT; s2 < K; s2 += NT) sslot[s2] = S.slot[s2];
+    if (tid < 6) ssgn[tid] = sgv;
+  }
+  __syncthreads();
+  for (int r = tid; r < Kpl * NE; r += NT) {
+    const int k = r / NE, le = r % NE;
+    T v = T(0);
+    if (le < nvalid && k < Kl) {
+      const ElemDesc& D = sd[le];
+      const int msk = sm[le];
+      const int code = sslot[k];
+      const int kind = code & 3, geo = (code >> 2) & 7, sub = code >> 5;
+      if (kind == 0) {
+        v = ((msk >> geo) & 1) ? T(0) : x[D.v[geo]];
+      } else if (kind == 1) {
+        const int ed = D.e[geo];
+        v = ((msk >> (4 + geo)) & 1) ? T(0) : x[(ed & 0x7fffffff) + sub];
+        if (ed < 0 && (sub & 1)) v = -v;
+      } else if (kind == 3) {
+        v = ((msk >> 14) & 1) ? T(0) : x[D.t + sub];
+      } else {
+        const int fo = D.fo[geo];
+        const bool fmask = (msk >> (10 + geo)) & 1;
+        if ((dg >> (fo & 7)) & 1) {
+          v = fmask ? T(0) : x[D.f[geo] + sub];
+          if (sub < 32 && ((ssgn[fo & 7] >> sub) & 1)) v = -v;
+        } else {
+          const int row = (fo & 7) * S.ndg2 + sub;
+          T acc = T(0);
+          const int nterm = cn[row];
+          for (int j = 0; j < nterm; ++j) {
+            const int key = ck[row * J + j];
+            const int ie = key & 3, idx = key >> 2;
+            T w = cw[row * J + j];
+            T xq;
+            if (ie < 3) {
+              const int le2 = (fo >> (3 + 3 * ie)) & 7;
+              if (((fo >> (12 + ie)) & 1) && (idx & 1)) w = -w;
+              xq = ((msk >> (4 + le2)) & 1) ? T(0) : x[(D.e[le2] & 0x7fffffff) + idx];
+            } else {
+              xq = fmask ? T(0) : x[D.f[geo] + idx];
+            }
+            acc += xq * w;
+          }
+          v = acc;
+        }
+      }
+    }
+    X[(size_t)k * VS + le] = v;
+  }
+  __syncthreads();
+
+  // ---- GEMM 1 + mixing into registers: wave w owns m tiles w, w + WAVES
+  T vv[TPW][CT][3][4];
+#pragma unroll
+  for (int tp = 0; tp < TPW; ++tp) {
+    const int t = wave + WAVES * tp;
+    if (t >= Mt) break;
+    v4 acc[3][CT];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int c = 0; c < CT; ++c) acc[a][c] = v4{0, 0, 0, 0};
+    const T* d0 = Dp + (size_t)(0 * Mp + 16 * t + lr) * Cp + lk;
+    const T* d1 = Dp + (size_t)(1 * Mp + 16 * t + lr) * Cp + lk;
+    const T* d2 = Dp + (size_t)(2 * Mp + 16 * t + lr) * Cp + lk;
+    const int kb = S.kbeg1[t];
+#pragma unroll 2
+    for (int k = kb; k < Kpl; k += 4) {
+      const T a0 = d0[k], a1 = d1[k], a2 = d2[k];
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        const T b = X[(size_t)(k + lk) * VS + 16 * c + lr];
+        acc[0][c] = Mma<T>::op(a0, b, acc[0][c]);
+        acc[1][c] = Mma<T>::op(a1, b, acc[1][c]);
+        acc[2][c] = Mma<T>::op(a2, b, acc[2][c]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const int col = 16 * c + lr, e = e0 + col;
+      T g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0, g5 = 0;
+      if (e < S.nt) {
+        const T* g = (sizeof(T) == 8) ? (const T*)(S.ghat + 6 * (size_t)e) : (const T*)(S.ghat32 + 6 * (size_t)e);
+        g0 = cst * g[0]; g1 = cst * g[1]; g2 = cst * g[2]; g3 = cst * g[3]; g4 = cst * g[4]; g5 = cst * g[5];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int m = 16 * t + Mma<T>::row(lk, j);
+        const T u0 = acc[0][c][j], u1 = acc[1][c][j], u2 = acc[2][c][j];
+        const T keep = (m < Kdl) ? T(1) : T(0);
+        vv[tp][c][0][j] = keep * (g0 * u0 + g3 * u1 + g4 * u2);
+        vv[tp][c][1][j] = keep * (g3 * u0 + g1 * u1 + g5 * u2);
+        vv[tp][c][2][j] = keep * (g4 * u0 + g5 * u1 + g2 * u2);
+      }
+    }
+  }
+  __syncthreads();  // every wave is done reading X: V may take its place
+#pragma unroll
+  for (int tp = 0; tp < TPW; ++tp) {
+    const int t = wave + WAVES * tp;
+    if (t >= Mt) break;
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          V[(size_t)(a * Mp + 16 * t + Mma<T>::row(lk, j)) * VS + 16 * c + lr] = vv[tp][c][a][j];
+  }
+  __syncthreads();
+
+  // ---- GEMM 2 and the Y write, as in k_elem_gemm
+  constexpr int RJ = (RMAX + WAVES - 1) / WAVES;
+  v4 acc[RJ][CT];
+#pragma unroll
+  for (int j = 0; j < RJ; ++j)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[j][c] = v4{0, 0, 0, 0};
+  int kend[RJ];
+  int kmax = 0;
+#pragma unroll
+  for (int j = 0; j < RJ; ++j) {
+    const int t2 = wave + WAVES * j;
+    kend[j] = (t2 < Rt) ? min(S.kend2[t2], 16 * Mt) : 0;
+    kmax = max(kmax, kend[j]);
+  }
+  for (int a = 0; a < 3; ++a) {
+    for (int k0 = 0; k0 < kmax; k0 += 16) {
+      T av[4][RJ], bv[4][CT];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + 4 * u;
+#pragma unroll
+        for (int j = 0; j < RJ; ++j) {
+          const int t2 = wave + WAVES * j;
+          const int i = 16 * (t2 < Rt ? t2 : 0) + lr;
+          av[u][j] = (k < kmax) ? Dp[(size_t)(a * Mp + k + lk) * Cp + i] : T(0);
+        }
+#pragma unroll
+        for (int c = 0; c < CT; ++c) bv[u][c] = (k < kmax) ? V[(size_t)(a * Mp + k + lk) * VS + 16 * c + lr] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + 4 * u;
+#pragma unroll
+        for (int j = 0; j < RJ; ++j)
+          if (k < kend[j]) {
+#pragma unroll
+            for (int c = 0; c < CT; ++c) acc[j][c] = Mma<T>::op(av[u][j], bv[u][c], acc[j][c]);
+          }
+      }
+    }
+  }
+  __syncthreads();  // every wave is done reading V
+  T* Y = V;         // [K][VS]
+#pragma unroll
+  for (int j = 0; j < RJ; ++j) {
+    const int t2 = wave + WAVES * j;
+    if (t2 < Rt) {
+#pragma unroll
+      for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int i = 16 * t2 + Mma<T>::row(lk, q);
+          if (i < K) Y[(size_t)i * VS + 16 * c + lr] = acc[j][c][q];
+        }
+    }
+  }
+  __syncthreads();
+  for (int s2 = tid; s2 < nvalid * K; s2 += NT) {
+    const int le = s2 / K, i = s2 % K;
+    yl[(size_t)(e0 + le) * K + i] = (i < Kl && i < 16 * Rt) ? Y[(size_t)i * VS + le] : T(0);
+  }
+}
+
 // Mass term on the matrix cores: Y (K x NE) = M (K x K) X (K x NE) with the
 // reference mass matrix M = mfac^T mfac (Cp x Cp, zero padded), column e
 // scaled by mass 6|V_e|; the same operand layout as GEMM 1 above.  Wave w
@@ -435,6 +694,17 @@ static size_t gemm_lds(const DevSpace& S) {
   return rows * (16 * CT + 1) * sizeof(T);
 }
 
+// LDS of the fused kernel: the V / Y tile, or the X tile with the staging
+// tables, whichever is larger
+template <typename T, int CT>
+static size_t gemm_gx_lds(const DevSpace& S) {
+  const size_t NE = 16 * CT, VS = NE + 1, R = 6 * (size_t)S.ndg2, J = S.ftc_J;
+  const size_t stage = (size_t)S.Kp * VS * sizeof(T) + NE * sizeof(ElemDesc) + (NE + (NE & 1)) * 4 +
+                       R * J * (sizeof(T) + 4) + R * 4 + (size_t)S.K * 4 + 24;
+  const size_t v = gemm_lds<T, CT>(S);
+  return v > stage ? v : stage;
+}
+
 bool mfma_supported(const DevSpace& S) {
   return S.Dp64 != nullptr && S.Mm64 != nullptr && S.xt64 != nullptr && S.K <= 16 * RMAX &&
          gemm_lds<float, 4>(S) <= 160 * 1024 && gemm_lds<double, 2>(S) <= 160 * 1024 &&
@@ -449,12 +719,19 @@ static void run(const DevSpace& S, const OpCoef& op, const T* Dp, const T* Mm, T
   const int Kl = op.Kl > 0 ? op.Kl : S.K;
   const bool timed = op.timed();
   const int kind = sizeof(T) == 8 ? TK_GATHER64 : TK_GATHER32;
+  int nb = (S.nt + 16 * CT - 1) / (16 * CT);
+  const bool mass = op.mass != 0.0, stiff = op.stiff != 0.0 || !mass;
+  if (stiff && !mass && S.Mp <= 128 && gemm_gx_lds<T, CT>(S) <= 160 * 1024) {
+    // stiffness alone: the gather fused into the element kernel (timed as it)
+    const int th1 = timed ? timing_begin(kind + 1, st) : -1;
+    hipLaunchKernelGGL((k_elem_gemm_gx<T, CT>), dim3(nb), dim3(NT), (gemm_gx_lds<T, CT>(S)), st, S, op, Dp, x, yl);
+    timing_end(th1, st);
+    return;
+  }
   int th = timed ? timing_begin(kind, st) : -1;
   const size_t ftc = (size_t)6 * S.ndg2 * (S.ftc_J * (sizeof(T) + 4) + 4) + (size_t)S.K * 4;
   hipLaunchKernelGGL(k_gather_tiles<T>, dim3((unsigned)S.ntile), dim3(256), ftc, st, S, Kl, x, xt, op.constrained);
   timing_end(th, st);
-  int nb = (S.nt + 16 * CT - 1) / (16 * CT);
-  const bool mass = op.mass != 0.0, stiff = op.stiff != 0.0 || !mass;
   th = timed ? timing_begin(kind + 1, st) : -1;
   if (stiff) hipLaunchKernelGGL((k_elem_gemm<T, CT>), dim3(nb), dim3(NT), (gemm_lds<T, CT>(S)), st, S, op, Dp, xt, yl);
   if (mass)
