@@ -25,6 +25,8 @@
 // (measured on gfx950).
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include "kernels.hpp"
 
 namespace tsem {
@@ -382,6 +384,8 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_gx(DevSpace S, OpCoef op, c
   const int e0 = blockIdx.x * NE;
   const int nvalid = min(NE, S.nt - e0);
   const T cst = (T)op.stiff;
+  x += (size_t)blockIdx.y * op.ldx;  // column of a block apply
+  yl += (size_t)blockIdx.y * op.ldl;
   // ---- phase 0: staging and the gather into X
   const int R = 6 * S.ndg2, J = S.ftc_J;
   T* X = reinterpret_cast<T*>(smem_raw);  // [Kp][VS]
@@ -711,6 +715,11 @@ bool mfma_supported(const DevSpace& S) {
          mass_lds<float, 4>(S) <= 160 * 1024 && mass_lds<double, 2>(S) <= 160 * 1024;
 }
 
+bool mfma_gx_supported(const DevSpace& S) {
+  return mfma_supported(S) && S.Mp <= 128 && gemm_gx_lds<float, 4>(S) <= 160 * 1024 &&
+         gemm_gx_lds<double, 2>(S) <= 160 * 1024;
+}
+
 // stiffness (k_elem_gemm) and mass (k_mass_gemm) terms from one gather; an
 // operator with neither runs the stiffness kernel, which then writes zeros
 template <typename T, int CT>
@@ -721,13 +730,16 @@ static void run(const DevSpace& S, const OpCoef& op, const T* Dp, const T* Mm, T
   const int kind = sizeof(T) == 8 ? TK_GATHER64 : TK_GATHER32;
   int nb = (S.nt + 16 * CT - 1) / (16 * CT);
   const bool mass = op.mass != 0.0, stiff = op.stiff != 0.0 || !mass;
-  if (stiff && !mass && S.Mp <= 128 && gemm_gx_lds<T, CT>(S) <= 160 * 1024) {
-    // stiffness alone: the gather fused into the element kernel (timed as it)
+  if (stiff && !mass && mfma_gx_supported(S)) {
+    // stiffness alone: the gather fused into the element kernel (timed as
+    // it), over op.ncol columns
     const int th1 = timed ? timing_begin(kind + 1, st) : -1;
-    hipLaunchKernelGGL((k_elem_gemm_gx<T, CT>), dim3(nb), dim3(NT), (gemm_gx_lds<T, CT>(S)), st, S, op, Dp, x, yl);
+    hipLaunchKernelGGL((k_elem_gemm_gx<T, CT>), dim3(nb, op.ncol), dim3(NT), (gemm_gx_lds<T, CT>(S)), st, S, op, Dp,
+                       x, yl);
     timing_end(th1, st);
     return;
   }
+  if (op.ncol != 1) throw std::logic_error("a block of columns needs the stiffness operator alone");
   int th = timed ? timing_begin(kind, st) : -1;
   const size_t ftc = (size_t)6 * S.ndg2 * (S.ftc_J * (sizeof(T) + 4) + 4) + (size_t)S.K * 4;
   hipLaunchKernelGGL(k_gather_tiles<T>, dim3((unsigned)S.ntile), dim3(256), ftc, st, S, Kl, x, xt, op.constrained);

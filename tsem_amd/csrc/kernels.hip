@@ -548,10 +548,18 @@ static int apply_path_from_env() {
 }
 static int g_apply_path = apply_path_from_env();
 void set_apply_path(int path) { g_apply_path = path; }
-bool block_apply_supported(const DevSpace& S, const OpCoef& op) {
-  return op.ncol >= 1 && op.mass == 0.0 && op.pot == 0.0 && g_apply_path == 0 && mfma_h_supported(S);
-}
 int apply_path() { return g_apply_path; }
+bool split_path(const DevSpace& S, const OpCoef& op) {
+  if (!(op.mass == 0.0 && g_apply_path == 0 && mfma_h_supported(S))) return false;
+  // measured on the 32^3 mesh, element kernel + scatter per apply: p = 2: 65
+  // (split) against 50 us (MFMA), p = 3: 86 / 76, p = 4: 122 / 117, p = 5:
+  // 176 / 182, p = 6: 234 / 319
+  return op.pot != 0.0 || S.K > 35 || !mfma_gx_supported(S);
+}
+bool block_apply_supported(const DevSpace& S, const OpCoef& op) {
+  return op.ncol >= 1 && op.mass == 0.0 && op.pot == 0.0 && g_apply_path == 0 &&
+         (split_path(S, op) || (S.K <= 35 && mfma_gx_supported(S)));
+}
 
 template <typename T>
 static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, double* partial, hipStream_t st) {
@@ -560,7 +568,13 @@ static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T*
     // a block of columns: the fp16-split element kernel and the scatter,
     // each one launch over (blocks, columns)
     if (!(sizeof(T) == 4 && block_apply_supported(S, op)))
-      throw std::logic_error("block apply needs the fp16-split FP32 path without mass or potential terms");
+      throw std::logic_error("block apply needs the FP32 stiffness operator on the default path");
+    if (!split_path(S, op)) {
+      // orders up to 4: the FP32 MFMA kernel, element-major results
+      launch_apply_mfma<T>(S, op, x, yl, st);
+      scatter_dispatch<T>(S, yl, y, x, op.constrained, partial, st, op.ncol, op.ldl, op.ldx);
+      return;
+    }
     const bool slab = g_slab && S.sl_meta && op.Kl == 0 && slab_scatter_fits(S) && slab_h_fits(S);
     launch_apply_mfma_h(S, op, (const float*)x, (float*)yl, slab, st);
     if (slab) slab_scatter<T>(S, yl, y, x, op.constrained, partial, st, op.ncol, op.ldl, op.ldx);
@@ -568,7 +582,7 @@ static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T*
     return;
   }
   bool slab = false;
-  if (sizeof(T) == 4 && op.mass == 0.0 && g_apply_path == 0 && mfma_h_supported(S)) {
+  if (sizeof(T) == 4 && split_path(S, op)) {
     // entity-slab results when the element kernel's output goes straight to
     // the scatter (full order, no potential term added to it)
     slab = g_slab && S.sl_meta && op.Kl == 0 && op.pot == 0.0 && slab_scatter_fits(S) && slab_h_fits(S);

@@ -148,6 +148,8 @@ void launch_apply(const DevSpace& S, const OpCoef& op, const T* x, T* y, T* yl, 
 template <typename T>
 void launch_apply_mfma(const DevSpace& S, const OpCoef& op, const T* x, T* yl, hipStream_t st);
 bool mfma_supported(const DevSpace& S);
+// the MFMA stiffness kernel with its gather fused in (k_elem_gemm_gx)
+bool mfma_gx_supported(const DevSpace& S);
 // fp16-split path for FP32 vectors (x: FP32 DOF vector, yl: element results,
 // element-major, or the entity slab when slab is set)
 void launch_apply_mfma_h(const DevSpace& S, const OpCoef& op, const float* x, float* yl, bool slab, hipStream_t st);
@@ -183,6 +185,12 @@ int apply_path();
 // the FP32 operator on a block of columns in one launch per kernel
 // (OpCoef::ncol): fp16-split path, stiffness only, default apply path
 bool block_apply_supported(const DevSpace& S, const OpCoef& op);
+// the FP32 operator of op runs on the fp16-split element kernel (else on the
+// FP32 MFMA kernel or the scalar one): stiffness (and potential) operators on
+// the default path, except orders up to 4 (K <= 35), where the FP32 MFMA
+// kernel with its fused gather is faster (and exact FP32); a block of
+// columns takes the same kernel as each column alone
+bool split_path(const DevSpace& S, const OpCoef& op);
 // captured-graph replay of PCG chunks and V-cycles on (tsem_set_graphs)
 bool graphs_enabled();
 // the eigen solvers' preconditioner as one block V-cycle per iteration

@@ -61,7 +61,7 @@ static size_t scratch_layout(DevSpace& d, const OpCoef& op, char* base, T** yl) 
   const size_t tiles = (size_t)d.ntile * TILE;
   size_t off = align256(tiles * d.K * sizeof(T));  // results: at most one value per (element, mode)
   if (base) *yl = reinterpret_cast<T*>(base);
-  if (sizeof(T) == 4 && op.mass == 0.0 && apply_path() == 0 && mfma_h_supported(d)) {
+  if (sizeof(T) == 4 && split_path(d, op)) {
     // the fp16-split kernel gathers into LDS: results only
   } else if (op.mass == 0.0 && (apply_path() == 0 || apply_path() == 2) && mfma_supported(d)) {
     const size_t t = tiles * d.Kp * sizeof(T);
