@@ -242,10 +242,25 @@ __device__ inline double block_sum(double v, double* red) {
 // adds in order.  The optional partial sums of x.y (one per block) feed the
 // CG denominator.  The grid is reduce_blocks(ndof) with a fixed row-to-block
 // assignment.
+// the scatter's result for DOF g: y[g] = acc, or the fused Chebyshev step
+// (ChebFuse; xg = x[g], the apply's input; column offset o)
 template <typename T>
-__global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict__ yl, T* __restrict__ y,
-                                                 const T* __restrict__ xin, int constrained,
-                                                 double* __restrict__ partial, long ldl, long ldy) {
+__device__ inline void scatter_out(const ChebFuse& cf, T* y, size_t o, int g, T acc, T xg) {
+  if (!cf.b) {
+    y[g] = acc;
+    return;
+  }
+  const T r = static_cast<const T*>(cf.b)[o + g] - acc;
+  T dn = T(cf.c2) * static_cast<const T*>(cf.inv)[g] * r;
+  if (cf.c1 != 0.0) dn += T(cf.c1) * static_cast<T*>(cf.dv)[o + g];
+  static_cast<T*>(cf.dv)[o + g] = dn;
+  y[g] = xg + dn;  // y is x here
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict__ yl, T* y, const T* xin,
+                                                 int constrained, double* __restrict__ partial, long ldl, long ldy,
+                                                 ChebFuse cf) {
 #pragma clang fp contract(off)
   yl += (size_t)blockIdx.y * ldl;  // column of a block apply
   y += (size_t)blockIdx.y * ldy;
@@ -283,9 +298,10 @@ __global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict
         }
       }
     }
-    if (constrained && S.cmask[g]) acc = (T)S.diag[g] * xin[g];
-    y[g] = acc;
-    if (partial) d += (double)xin[g] * (double)acc;
+    const T xg = (partial || constrained || cf.b) ? xin[g] : T(0);
+    if (constrained && S.cmask[g]) acc = (T)S.diag[g] * xg;
+    scatter_out(cf, y, (size_t)blockIdx.y * ldy, g, acc, xg);
+    if (partial) d += (double)xg * (double)acc;
   }
   if (!partial) return;
   double tot = block_sum(d, red);
@@ -302,9 +318,9 @@ __global__ void __launch_bounds__(BLK) k_scatter(DevSpace S, const T* __restrict
 constexpr int SCH = BLK;
 static_assert(SCH == SLAB_ROWS, "one slab chunk per scatter block");
 template <typename T>
-__global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __restrict__ yl, T* __restrict__ y,
-                                                        const T* __restrict__ xin, int constrained,
-                                                        double* __restrict__ partial, long ldl, long ldy) {
+__global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __restrict__ yl, T* y, const T* xin,
+                                                        int constrained, double* __restrict__ partial, long ldl,
+                                                        long ldy, ChebFuse cf) {
 #pragma clang fp contract(off)
   yl += (size_t)blockIdx.y * ldl;  // column of a block apply
   y += (size_t)blockIdx.y * ldy;
@@ -320,7 +336,7 @@ __global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __r
   const int xcd = blockIdx.x % NXCD, jb = blockIdx.x / NXCD;
   const int nbx = gridDim.x / NXCD + (xcd < (int)(gridDim.x % NXCD) ? 1 : 0);
   const int c0 = (int)((long long)xcd * nch / nx), c1 = (int)((long long)(xcd + 1) * nch / nx);
-  const bool need_x = partial || constrained;
+  const bool need_x = partial || constrained || cf.b;
   for (int c = c0 + jb; c < c1; c += nbx) {
     const int g0 = c * SCH, g1 = min(S.ndof, g0 + SCH);
     const int t0 = S.sc_ptr[g0], t1 = S.sc_ptr[g1];
@@ -354,7 +370,7 @@ __global__ void __launch_bounds__(BLK) k_scatter_chunks(DevSpace S, const T* __r
       T acc = T(0);
       for (int t = a; t < b; ++t) acc += vals[t];
       if (constrained && cg) acc = (T)S.diag[g] * xg;
-      y[g] = acc;
+      scatter_out(cf, y, (size_t)blockIdx.y * ldy, g, acc, xg);
       if (partial) d += (double)xg * (double)acc;
     }
     __syncthreads();  // vals is refilled by the next chunk
@@ -378,9 +394,9 @@ constexpr int SP = 4;   // chunks per step
 static_assert(SP % 2 == 0 && SLAB_CHUNK_GROUP % SP == 0, "a step reads whole int4 pairs of its group");
 constexpr int SU = 4;   // span entries per thread and chunk: spans <= SU * BLK
 template <typename T>
-__global__ void __launch_bounds__(BLK) k_scatter_slab(DevSpace S, const T* __restrict__ sl, T* __restrict__ y,
-                                                      const T* __restrict__ xin, int constrained,
-                                                      double* __restrict__ partial, long ldl, long ldy) {
+__global__ void __launch_bounds__(BLK) k_scatter_slab(DevSpace S, const T* __restrict__ sl, T* y, const T* xin,
+                                                      int constrained, double* __restrict__ partial, long ldl,
+                                                      long ldy, ChebFuse cf) {
   sl += (size_t)blockIdx.y * ldl;  // column of a block apply
   y += (size_t)blockIdx.y * ldy;
   if (xin) xin += (size_t)blockIdx.y * ldy;
@@ -395,7 +411,7 @@ __global__ void __launch_bounds__(BLK) k_scatter_slab(DevSpace S, const T* __res
   const int nbx = gridDim.x / NXCD + (xcd < (int)(gridDim.x % NXCD) ? 1 : 0);
   const int q0 = (int)((long long)xcd * ngr / nx), q1 = (int)((long long)(xcd + 1) * ngr / nx);
   const int n1 = S.sl_n1, n2 = S.sl_n2, cm = S.sl_chunk_max;
-  const bool need_x = partial || constrained;
+  const bool need_x = partial || constrained || cf.b;
   // span bounds of the step about to run; each step loads the next step's
   // (the compiler places that load at the end of the step).  The table is
   // padded with empty chunks to whole groups, so a step's bounds are one
@@ -452,7 +468,7 @@ __global__ void __launch_bounds__(BLK) k_scatter_slab(DevSpace S, const T* __res
         T acc = T(0);
         for (int k = 0; k < m; ++k) acc += row[k * stride];
         if (constrained && cg[p]) acc = (T)S.diag[g] * xg[p];
-        y[g] = acc;
+        scatter_out(cf, y, (size_t)blockIdx.y * ldy, g, acc, xg[p]);
         if (partial) d += (double)xg[p] * (double)acc;
       }
     }
@@ -478,23 +494,25 @@ bool slab_enabled() { return g_slab; }
 // apart), with no partial sums
 template <typename T>
 static void slab_scatter(const DevSpace& S, const T* sl, T* y, const T* x, int constrained, double* partial,
-                         hipStream_t st, int ncol = 1, long ldl = 0, long ldy = 0) {
+                         hipStream_t st, int ncol = 1, long ldl = 0, long ldy = 0, const ChebFuse& cf = ChebFuse()) {
   if (ncol > 1 && partial) throw std::logic_error("block scatter takes no partial sums");
   hipLaunchKernelGGL(k_scatter_slab<T>, dim3(reduce_blocks(S.ndof), ncol), dim3(BLK),
-                     (size_t)SP * S.sl_chunk_max * sizeof(T), st, S, sl, y, x, constrained, partial, ldl, ldy);
+                     (size_t)SP * S.sl_chunk_max * sizeof(T), st, S, sl, y, x, constrained, partial, ldl, ldy, cf);
 }
 
 // the chunked scatter when a chunk's terms fit in LDS, else the row kernel
 template <typename T>
 static void scatter_dispatch(const DevSpace& S, const T* yl, T* y, const T* x, int constrained, double* partial,
-                             hipStream_t st, int ncol = 1, long ldl = 0, long ldy = 0) {
+                             hipStream_t st, int ncol = 1, long ldl = 0, long ldy = 0,
+                             const ChebFuse& cf = ChebFuse()) {
   if (ncol > 1 && partial) throw std::logic_error("block scatter takes no partial sums");
   const dim3 grid(reduce_blocks(S.ndof), ncol);
   size_t lds = (size_t)S.sc_chunk_max * sizeof(T);
   if (S.sc_chunk_max > 0 && lds <= 64 * 1024)
-    hipLaunchKernelGGL(k_scatter_chunks<T>, grid, dim3(BLK), lds, st, S, yl, y, x, constrained, partial, ldl, ldy);
+    hipLaunchKernelGGL(k_scatter_chunks<T>, grid, dim3(BLK), lds, st, S, yl, y, x, constrained, partial, ldl, ldy,
+                       cf);
   else
-    hipLaunchKernelGGL(k_scatter<T>, grid, dim3(BLK), 0, st, S, yl, y, x, constrained, partial, ldl, ldy);
+    hipLaunchKernelGGL(k_scatter<T>, grid, dim3(BLK), 0, st, S, yl, y, x, constrained, partial, ldl, ldy, cf);
 }
 
 // ---- optional per-kernel timing with HIP events on the launching stream ----
@@ -572,13 +590,13 @@ static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T*
     if (!split_path(S, op)) {
       // orders up to 4: the FP32 MFMA kernel, element-major results
       launch_apply_mfma<T>(S, op, x, yl, st);
-      scatter_dispatch<T>(S, yl, y, x, op.constrained, partial, st, op.ncol, op.ldl, op.ldx);
+      scatter_dispatch<T>(S, yl, y, x, op.constrained, partial, st, op.ncol, op.ldl, op.ldx, op.cheb);
       return;
     }
     const bool slab = g_slab && S.sl_meta && op.Kl == 0 && slab_scatter_fits(S) && slab_h_fits(S);
     launch_apply_mfma_h(S, op, (const float*)x, (float*)yl, slab, st);
-    if (slab) slab_scatter<T>(S, yl, y, x, op.constrained, partial, st, op.ncol, op.ldl, op.ldx);
-    else scatter_dispatch<T>(S, yl, y, x, op.constrained, partial, st, op.ncol, op.ldl, op.ldx);
+    if (slab) slab_scatter<T>(S, yl, y, x, op.constrained, partial, st, op.ncol, op.ldl, op.ldx, op.cheb);
+    else scatter_dispatch<T>(S, yl, y, x, op.constrained, partial, st, op.ncol, op.ldl, op.ldx, op.cheb);
     return;
   }
   bool slab = false;
@@ -604,8 +622,8 @@ static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T*
     hipLaunchKernelGGL(k_apply_potential<T>, dim3(S.nt), dim3(BLK), lds, st, S, op.pot, x, yl, op.constrained);
   }
   int th = op.timed() ? timing_begin(kind + 2, st) : -1;
-  if (slab) slab_scatter<T>(S, yl, y, x, op.constrained, partial, st);
-  else scatter_dispatch<T>(S, yl, y, x, op.constrained, partial, st);
+  if (slab) slab_scatter<T>(S, yl, y, x, op.constrained, partial, st, 1, 0, 0, op.cheb);
+  else scatter_dispatch<T>(S, yl, y, x, op.constrained, partial, st, 1, 0, 0, op.cheb);
   timing_end(th, st);
 }
 

@@ -120,6 +120,17 @@ struct DevSpace {
   const float* sl_tw32 = nullptr;
 };
 
+// A Chebyshev step fused into the scatter of an apply (p-multigrid
+// smoothing): in place of y = A x the scatter forms d = c1 d + c2 Dinv (b - A x)
+// and x += d, x being the apply's input (updated in place, each DOF by the one
+// thread that owns it); with c1 = 0, d is not read.  Unused when b is null.
+struct ChebFuse {
+  const void* b = nullptr;
+  const void* inv = nullptr;  // Dinv, one value per DOF (shared by the columns)
+  void* dv = nullptr;
+  double c1 = 0.0, c2 = 0.0;
+};
+
 // operator selection
 struct OpCoef {
   double stiff = 1.0, mass = 0.0;
@@ -132,6 +143,7 @@ struct OpCoef {
   // fp16-split FP32 path takes ncol > 1 (the p-multigrid block V-cycle)
   int ncol = 1;
   long ldx = 0, ldl = 0;
+  ChebFuse cheb;  // scatter forms a Chebyshev step instead of y (cheb.b set)
   // applies counted by the per-kernel timing: the space's own full-order operator
   bool timed() const { return Kl == 0 && level == 0; }
 };

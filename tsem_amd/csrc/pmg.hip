@@ -390,9 +390,10 @@ void Pmg::block_reserve(int ncol) {
 }
 
 template <typename T>
-void Pmg::apply_level(int l, const T* x, T* y, hipStream_t st, int ncol) {
+void Pmg::apply_level(int l, const T* x, T* y, hipStream_t st, int ncol, const ChebFuse& cf) {
   const int n = (int)ndof_level[l];
   if (l == csr_level) {
+    if (cf.b) throw std::logic_error("the assembled level takes no fused Chebyshev step");
     const T* val = sizeof(T) == 8 ? (const T*)csr_v64 : (const T*)csr_v32;
     hipLaunchKernelGGL(k_csr_spmv<T>, dim3(nblk(n), ncol), dim3(BLK), 0, st, n, (const int32_t*)csr_rs,
                        (const int32_t*)csr_col, val, x, y);
@@ -401,6 +402,7 @@ void Pmg::apply_level(int l, const T* x, T* y, hipStream_t st, int ncol) {
   OpCoef op;
   op.constrained = 1;
   op.level = l;
+  op.cheb = cf;
   tsem_space* s = l == 0 ? fine : sub[l - 1];
   T* yl = sizeof(T) == 8 ? (T*)s->yl64.p : (T*)s->yl32.p;
   if (ncol > 1) {
@@ -444,14 +446,24 @@ void Pmg::smooth_level(int l, T* x, const T* b, int degree, bool zero_init, hipS
     }
     return;
   }
-  if (!zero_init) apply_level<T>(l, x, Ax, st, ncol);
-  hipLaunchKernelGGL(k_cheb<T>, grid, dim3(BLK), 0, st, n, b, zero_init ? (const T*)nullptr : (const T*)Ax, inv, T(0),
-                     T(1.0 / theta), dv, x);
+  // steps after a product: the step is formed by the apply's scatter
+  // (ChebFuse), with no A x vector and no separate pass over the vectors
+  auto fused = [&](double c1, double c2) {
+    ChebFuse cf;
+    cf.b = b;
+    cf.inv = inv;
+    cf.dv = dv;
+    cf.c1 = c1;
+    cf.c2 = c2;
+    apply_level<T>(l, x, x, st, ncol, cf);
+  };
+  if (zero_init)
+    hipLaunchKernelGGL(k_cheb<T>, grid, dim3(BLK), 0, st, n, b, (const T*)nullptr, inv, T(0), T(1.0 / theta), dv, x);
+  else
+    fused(0.0, 1.0 / theta);
   for (int k = 1; k < degree; ++k) {
-    apply_level<T>(l, x, Ax, st, ncol);
     const double rho = 1.0 / (2 * sigma - rho_old);
-    hipLaunchKernelGGL(k_cheb<T>, grid, dim3(BLK), 0, st, n, b, (const T*)Ax, inv, T(rho * rho_old),
-                       T(2 * rho / delta), dv, x);
+    fused(rho * rho_old, 2 * rho / delta);
     rho_old = rho;
   }
 }

@@ -85,7 +85,7 @@ class Pmg {
   void block_reserve(int ncol);
   long yl_cols(int l) const;  // FP32 element-result entries of one column
   template <typename T>
-  void apply_level(int l, const T* x, T* y, hipStream_t st, int ncol = 1);
+  void apply_level(int l, const T* x, T* y, hipStream_t st, int ncol = 1, const ChebFuse& cf = ChebFuse());
   template <typename T>
   void smooth_level(int l, T* x, const T* b, int degree, bool zero_init, hipStream_t st, int ncol = 1);
 };
