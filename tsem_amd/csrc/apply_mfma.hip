@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_gx(DevSpace S, OpCoef op, c
   const int Kpl = (Kl + 3) & ~3;
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lk = lane >> 4;
-  const int e0 = blockIdx.x * NE;
+  const int e0 = xcd_block(blockIdx.x, gridDim.x) * NE;  // contiguous element range per XCD
   const int nvalid = min(NE, S.nt - e0);
   const T cst = (T)op.stiff;
   x += (size_t)blockIdx.y * op.ldx;  // column of a block apply

@@ -91,7 +91,10 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
   // vector-memory wait for them holds back the loads in flight
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lk = lane >> 4;
-  const int e0 = blockIdx.x * NE;
+  // each XCD takes a contiguous range of element blocks (xcd_block), so the
+  // DOFs that neighbouring blocks share are gathered from, and their slab
+  // lines written through, one L2
+  const int e0 = xcd_block(blockIdx.x, gridDim.x) * NE;
   const int KB = K32 / 8;
   const float cst = (float)op.stiff * S.dinv;  // undo the factor scaling of GEMM 1
   for (int s = tid; s < NE; s += NT) colmax[s] = 0u;
