@@ -135,7 +135,7 @@ def ks_section(ctx, log, n=24, p=5, steps=20, alpha=0.3):
     return out
 
 
-def serve_section(log, count=24, slots=8, seed=2026, loads=(0.8, 0.5)):
+def serve_section(log, count=24, slots=4, seed=2026, loads=(0.8, 0.5)):
     """Served Poisson stream: count requests (n in 8..32, p in 3..8, seeded
     jitter and problem), first all at once (saturated throughput, which also
     fills the space pool), then with Poisson arrivals at each of `loads` x

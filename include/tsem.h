@@ -408,7 +408,7 @@ typedef int (*tsem_value_fn)(void* user, const double* xyz, int64_t n, double* o
 typedef int (*tsem_flux_fn)(void* user, const double* xyz, const double* normal, int64_t n, double* out);
 
 typedef struct {
-  int slots;          /* worker threads, one context (stream) each; 0: 8 */
+  int slots;          /* worker threads, one context (stream) each; 0: 4 */
   int small_slots;    /* workers that take only small requests; -1: slots / 2 */
   double small_cost;  /* a request is small when n_elem * C(M+3,3) <= small_cost; 0: 4e6 */
   int64_t hbm_budget; /* device bytes the spaces in use and cached may hold; 0: 70% of free */

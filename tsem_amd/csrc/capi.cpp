@@ -79,8 +79,18 @@ struct ForegroundScope {
 };
 
 // max_seconds <= 0: until no foreground work is active
+// how long a background wait may last, as a multiple of the caller's
+// previous chunk (TSEM_BG_YIELD, default 1: background work keeps at least
+// half the device while foreground work runs)
+static const double g_bg_yield = [] {
+  const char* v = getenv("TSEM_BG_YIELD");
+  const double f = v ? atof(v) : 1.0;
+  return f > 0 ? f : 1.0;
+}();
+
 void background_wait(const tsem_ctx* c, double max_seconds) {
   if (!c || c->cls != 2) return;
+  if (max_seconds > 0) max_seconds *= g_bg_yield;
   std::unique_lock<std::mutex> lk(g_fg_mu);
   if (max_seconds <= 0) g_fg_cv.wait(lk, [] { return g_fg_active == 0; });
   else g_fg_cv.wait_for(lk, std::chrono::duration<double>(max_seconds), [] { return g_fg_active == 0; });

@@ -554,7 +554,11 @@ int tsem_serve_open(int device, const tsem_serve_opts* opts, tsem_server** out) 
     s->o.small_slots = -1;
     s->o.classes = 1;
   }
-  if (s->o.slots <= 0) s->o.slots = 8;
+  // 4 workers by default: at a fixed arrival rate, fewer requests in flight
+  // interfere less (2,000-request stream at 6.9 requests/s: latency over
+  // isolated p99 13.8 with 4 workers, 20.6 with 8, 29-43 with 16-24), for
+  // about 10% less saturated throughput
+  if (s->o.slots <= 0) s->o.slots = 4;
   if (s->o.slots > 64) throw std::invalid_argument("at most 64 workers");
   if (s->o.small_slots < 0) s->o.small_slots = s->o.slots / 2;
   if (s->o.small_slots > s->o.slots) throw std::invalid_argument("small_slots > slots");

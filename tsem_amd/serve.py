@@ -87,7 +87,7 @@ def make_requests(count, seed=0, n_range=(8, 32), p_range=(3, 8), rate=None):
 
 
 class Server:
-    def __init__(self, device=0, slots=8, small_slots=None, small_cost=None, hbm_budget=None, tol=1e-10,
+    def __init__(self, device=0, slots=4, small_slots=None, small_cost=None, hbm_budget=None, tol=1e-10,
                  keep_solutions=False, classes=True):
         """slots workers; the first small_slots of them (default half) take only
         requests whose cost is at most small_cost (default: the median over the
@@ -328,7 +328,7 @@ class NativeServer:
     (submit_eigen) and system requests on a caller's space (submit_system,
     e.g. a Hartree solve) share the workers and the lanes."""
 
-    def __init__(self, device=0, slots=8, small_slots=-1, small_cost=0.0, hbm_budget=0, classes=True):
+    def __init__(self, device=0, slots=4, small_slots=-1, small_cost=0.0, hbm_budget=0, classes=True):
         self.device = device
         o = ServeOpts(slots, small_slots, small_cost, hbm_budget, 1 if classes else 0)
         h = C.c_void_p()
@@ -483,7 +483,7 @@ class NativeServer:
             self.keep.clear()
 
 
-def stream(count=2000, load=0.8, slots=8, seed=2026, sat_count=300, log=print):
+def stream(count=2000, load=0.8, slots=4, seed=2026, sat_count=300, log=print, rate=None):
     """The served stream of the C ABI server at one offered load, measured
     end to end: count Poisson requests (n uniform in 8..32, p in 3..8, seed =
     request id, jittered Kuhn meshes, manufactured problems evaluated in the
@@ -529,7 +529,7 @@ def stream(count=2000, load=0.8, slots=8, seed=2026, sat_count=300, log=print):
         srv.poll(rid)
     sat_wall = time.perf_counter() - t
     sat_rate = len(ids) / sat_wall
-    rate = load * sat_rate
+    rate = load * sat_rate if not rate else rate
     log("stream: saturated %.2f requests/s over %d requests; offered %.2f requests/s" % (sat_rate, len(ids), rate))
     arrivals = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, count))
     sub = [None] * count
@@ -596,10 +596,11 @@ def main():
     ap = argparse.ArgumentParser(description="served Poisson stream on the C ABI server at one offered load")
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--load", type=float, default=0.8)
-    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--sat", type=int, default=300, help="requests of the saturated-rate measurement")
+    ap.add_argument("--rate", type=float, default=0.0, help="offered requests/s (default: load x saturated rate)")
     a = ap.parse_args()
-    out = stream(a.requests, a.load, a.slots, sat_count=a.sat,
+    out = stream(a.requests, a.load, a.slots, sat_count=a.sat, rate=a.rate or None,
                  log=lambda s: print("[serve]", s, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
 
