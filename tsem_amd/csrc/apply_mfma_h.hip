@@ -342,39 +342,47 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
   }
   __syncthreads();
 
-  // ---- phase C: GEMM 2, Y[i][e] = sum_r D[r][i] V[r][e].  Wave w owns row
-  // tiles w + WAVES j and every column tile: per 32-row step it issues all its
-  // factor loads (global) and the column tiles' V loads (LDS) at once, then
-  // the MFMAs, so each factor load serves all CT column tiles
+  // ---- phase C: GEMM 2, Y[i][e] = sum_r D[r][i] V[r][e].  Each wave owns
+  // RJ row tiles and NC = 2 column tiles: with CT = 2 every column and the
+  // row tiles w + WAVES j; with CT = 4 (K <= 96, at most 6 row tiles) the
+  // column tiles 2 (w & 1) + {0, 1} and the row tiles (w >> 1) + 2 j, so the
+  // six row tiles of p = 6 give every wave the same MFMA count (rows split
+  // 2-2-1-1 over whole column sets gave two waves twice the work).  Per
+  // 32-row step a wave issues all its factor loads (global) and V loads
+  // (LDS) at once, then the MFMAs
   constexpr int RJ = (RMAX + WAVES - 1) / WAVES;
-  f4 acc[RJ][CT];
+  constexpr int NC = 2;
+  static_assert(CT == 2 || CT == 4, "two or four column tiles");
+  auto tile2 = [&](int j) { return CT == 4 ? (wave >> 1) + 2 * j : wave + WAVES * j; };
+  auto ctile = [&](int cc) { return CT == 4 ? 2 * (wave & 1) + cc : cc; };
+  f4 acc[RJ][NC];
 #pragma unroll
   for (int j = 0; j < RJ; ++j)
 #pragma unroll
-    for (int c = 0; c < CT; ++c) acc[j][c] = f4{0, 0, 0, 0};
+    for (int c = 0; c < NC; ++c) acc[j][c] = f4{0, 0, 0, 0};
   int kend[RJ];
   int kmax = 0;
 #pragma unroll
   for (int j = 0; j < RJ; ++j) {
-    int t2 = wave + WAVES * j;
+    int t2 = tile2(j);
     kend[j] = (t2 < Rt) ? min(S.kend2h[t2], 16 * Mt) : 0;
     kmax = max(kmax, kend[j]);
   }
   for (int a = 0; a < 3; ++a) {
     for (int mm = 0; mm < kmax; mm += 32) {
       const int rr = a * Mh + mm + 8 * lk;
-      h8 ah[RJ], al[RJ], bh[CT], bl[CT];
+      h8 ah[RJ], al[RJ], bh[NC], bl[NC];
 #pragma unroll
       for (int j = 0; j < RJ; ++j) {
-        int t2 = wave + WAVES * j;
+        int t2 = tile2(j);
         int i = 16 * (t2 < Rt ? t2 : 0) + lr;
         size_t o = (size_t)i * S.R2g + rr;
         ah[j] = ld8(S.Dh2hi + o);
         al[j] = ld8(S.Dh2lo + o);
       }
 #pragma unroll
-      for (int c = 0; c < CT; ++c) {
-        int col = 16 * c + lr;
+      for (int c = 0; c < NC; ++c) {
+        int col = 16 * ctile(c) + lr;
         bh[c] = ld8(Vh + (size_t)col * R2 + rr);
         bl[c] = ld8(Vh + (size_t)(NE + col) * R2 + rr);
       }
@@ -382,7 +390,7 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
       for (int j = 0; j < RJ; ++j) {
         if (mm < kend[j]) {
 #pragma unroll
-          for (int c = 0; c < CT; ++c) {
+          for (int c = 0; c < NC; ++c) {
             acc[j][c] = mma(ah[j], bh[c], acc[j][c]);
             acc[j][c] = mma(ah[j], bl[c], acc[j][c]);
             acc[j][c] = mma(al[j], bh[c], acc[j][c]);
@@ -421,12 +429,12 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
     if (tid < 6) work[2 + tid] = (int32_t)S.sl_sgn[tid];
   }
 #pragma unroll
-  for (int c = 0; c < CT; ++c) {
-    const int col = 16 * c + lr;
+  for (int c = 0; c < NC; ++c) {
+    const int col = 16 * ctile(c) + lr;
     const float m = __uint_as_float(colmax[col]) * S.dinv;
 #pragma unroll
     for (int j = 0; j < RJ; ++j) {
-      int t2 = wave + WAVES * j;
+      int t2 = tile2(j);
       if (t2 < Rt) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
