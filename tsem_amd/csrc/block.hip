@@ -23,8 +23,8 @@ int gram_blocks(int n) {
 
 // partial[b p q + i q + j] = sum over the rows of block b of U[r][i] V[r][j].
 // Thread (ti, tj) of the 16 x 16 grid owns outputs GT ti .. GT ti + GT - 1 by
-// GT tj .. : GT = 2, 4, 6 serve operands of up to 32, 64, 96 columns, so a
-// narrow Gram does no work on padding.  The next chunk of rows is loaded
+// GT tj .. : GT = 4, 6 serve operands of up to 64, 96 columns (up to 32, the
+// matrix-core kernel below).  The next chunk of rows is loaded
 // into registers while the current one is multiplied out of LDS.
 template <int GT>
 __global__ void __launch_bounds__(256) k_gram(int n, int p, int q, const double* __restrict__ U, long ldu,
@@ -82,6 +82,87 @@ __global__ void __launch_bounds__(256) k_gram(int n, int p, int q, const double*
     }
 }
 
+// The same partial sums on the FP64 matrix cores, for operands of up to 32
+// columns: D (16 TI x 16 TJ) = U^T V over the block's rows, 16 rows a step.
+// Lane (lr, lk) loads rows r + 4 lk .. r + 4 lk + 3 of column 16 ti + lr of U
+// and of column 16 tj + lr of V (the four lanes of a column read one 128-byte
+// run), and step m of v_mfma_f64_16x16x4f64 takes row r + 4 lk + m as k index
+// lk of both operands.  Wave w takes the block's steps w, w + 4, ..., with the
+// next step's rows loaded while the current one is multiplied; the four
+// waves' tiles are added in wave order through LDS.  The sums have a fixed
+// order, so results are reproducible.  The vector kernel above spends its
+// time reading operands out of LDS (a 16 x 16 thread grid reads 2 GT values a
+// row for GT^2 FMAs); here each loaded value feeds 4 TJ (or 4 TI) MFMA lanes
+// straight from registers.
+template <int TI, int TJ>
+__global__ void __launch_bounds__(256) k_gram_mfma(int n, int p, int q, const double* __restrict__ U, long ldu,
+                                                   const double* __restrict__ V, long ldv, int rows_per_block,
+                                                   double* __restrict__ partial) {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  __shared__ double red[4][16 * TI][16 * TJ + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
+  const int r_begin = blockIdx.x * rows_per_block;
+  const int r_end = min(n, r_begin + rows_per_block);
+  v4 acc[TI][TJ];
+#pragma unroll
+  for (int a = 0; a < TI; ++a)
+#pragma unroll
+    for (int c = 0; c < TJ; ++c) acc[a][c] = v4{0, 0, 0, 0};
+  double a[TI][4], b[TJ][4], na[TI][4], nb[TJ][4];
+  auto fetch = [&](int r, double (&fa)[TI][4], double (&fb)[TJ][4]) {
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti) {
+      const int c = 16 * ti + lr;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int row = r + 4 * lk + m;
+        fa[ti][m] = (c < p && row < r_end) ? U[(size_t)c * ldu + row] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int tj = 0; tj < TJ; ++tj) {
+      const int c = 16 * tj + lr;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int row = r + 4 * lk + m;
+        fb[tj][m] = (c < q && row < r_end) ? V[(size_t)c * ldv + row] : 0.0;
+      }
+    }
+  };
+  int r = r_begin + 16 * wave;
+  if (r < r_end) fetch(r, a, b);
+  for (; r < r_end; r += 64) {
+    if (r + 64 < r_end) fetch(r + 64, na, nb);
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TJ; ++tj)
+          acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti][m], b[tj][m], acc[ti][tj], 0, 0, 0);
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) a[ti][m] = na[ti][m];
+#pragma unroll
+    for (int tj = 0; tj < TJ; ++tj)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) b[tj][m] = nb[tj][m];
+  }
+  // value v of acc[ti][tj] in lane (lr, lk): D[16 ti + lk + 4 v][16 tj + lr]
+#pragma unroll
+  for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < TJ; ++tj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) red[wave][16 * ti + lk + 4 * v][16 * tj + lr] = acc[ti][tj][v];
+  __syncthreads();
+  for (int s = tid; s < p * q; s += 256) {
+    const int i = s / q, j = s - i * q;
+    partial[(size_t)blockIdx.x * p * q + s] = ((red[0][i][j] + red[1][i][j]) + red[2][i][j]) + red[3][i][j];
+  }
+}
+
 // out[v] = sum_b partial[b nvals + v] in a fixed order: a block takes 32
 // values, lane group g (of 8) sums the partials b = g, g + 8, ... in
 // order, then the 8 group sums are added in group order.  Neighbouring
@@ -112,9 +193,17 @@ void launch_gram(int n, int p, int q, const double* U, long ldu, const double* V
   int rpb = (n + nb - 1) / nb;
   rpb = (rpb + GR - 1) / GR * GR;
   const int w = p > q ? p : q;
-  if (w <= 32)
-    hipLaunchKernelGGL(k_gram<2>, dim3(nb), dim3(256), 0, st, n, p, q, U, ldu, V, ldv, rpb, partial);
-  else if (w <= 64)
+  if (w <= 32) {
+    const int ti = p > 16 ? 2 : 1, tj = q > 16 ? 2 : 1;
+    if (ti == 1 && tj == 1)
+      hipLaunchKernelGGL((k_gram_mfma<1, 1>), dim3(nb), dim3(256), 0, st, n, p, q, U, ldu, V, ldv, rpb, partial);
+    else if (ti == 1)
+      hipLaunchKernelGGL((k_gram_mfma<1, 2>), dim3(nb), dim3(256), 0, st, n, p, q, U, ldu, V, ldv, rpb, partial);
+    else if (tj == 1)
+      hipLaunchKernelGGL((k_gram_mfma<2, 1>), dim3(nb), dim3(256), 0, st, n, p, q, U, ldu, V, ldv, rpb, partial);
+    else
+      hipLaunchKernelGGL((k_gram_mfma<2, 2>), dim3(nb), dim3(256), 0, st, n, p, q, U, ldu, V, ldv, rpb, partial);
+  } else if (w <= 64)
     hipLaunchKernelGGL(k_gram<4>, dim3(nb), dim3(256), 0, st, n, p, q, U, ldu, V, ldv, rpb, partial);
   else
     hipLaunchKernelGGL(k_gram<6>, dim3(nb), dim3(256), 0, st, n, p, q, U, ldu, V, ldv, rpb, partial);

@@ -111,6 +111,122 @@ __global__ void __launch_bounds__(BLK) k_apply_potential(DevSpace S, double pot,
   }
 }
 
+// M_V on the matrix cores (v_mfma_f64_16x16x4f64), 64 elements per block and
+// wave w owning elements 16 w .. 16 w + 15.  For each stage of 16 points,
+//   U^T (16 x 16) = Phi (16 points x K) X^T (K x 16 elements),
+//   C = U^T scaled by pot vol_e w_q V[e][q],
+//   Y (16 x K) += C^T (16 x 16) Phi (16 x K).
+// The first product's accumulator (value j of lane l: point l/16 + 4 j,
+// element l%16) is the second product's A operand as it stands: step j of the
+// second product takes point 4 j + l/16 as its k index in both operands.
+// X stays in registers (value s of lane l: element l%16, mode 4 s + l/16);
+// the stage's Phi rows and weighted point values sit in LDS.  Every point
+// costs each element 4 K FLOPs, which bounds the kernel by the FP64 matrix
+// rate (V, nq doubles per element, is the only large HBM stream).  Measured
+// at p = 5 (nq = 512) on 82,944 elements: stages of 16 points beat 32 and 64
+// (fewer registers, 3 waves per SIMD); gathering X through LDS, prefetching
+// the next stage into registers and a 4-wave register cap were all slower.
+typedef double pd4 __attribute__((ext_vector_type(4)));
+constexpr int POT_PE = 64, POT_RQ = 16;
+
+template <typename T, int MT>
+__global__ void __launch_bounds__(BLK) k_pot_gemm(DevSpace S, double pot, const T* __restrict__ x,
+                                                  T* __restrict__ yl, int constrained) {
+  constexpr int NW = 16 * MT, RQ = POT_RQ;
+  constexpr int PP = RQ * NW / BLK, PC = POT_PE * RQ / BLK;  // stage loads per thread
+  __shared__ double sP[RQ][NW + 1];      // phi_i(x_q) of the stage's points
+  __shared__ double sC[POT_PE][RQ + 1];  // pot vol_e w_q V[e][q]
+  __shared__ double sS[POT_PE];          // pot vol_e
+  const int K = S.K, nq = S.nq, nks = (K + 3) >> 2;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
+  const int eb = xcd_block(blockIdx.x, gridDim.x) * POT_PE;
+  const int ne = min(POT_PE, S.nt - eb);
+  const int ew = eb + 16 * wave + lr;
+  if (tid < POT_PE) sS[tid] = (tid < ne) ? pot * S.sixv[eb + tid] / 6.0 : 0.0;
+  double xr[4 * MT];
+#pragma unroll
+  for (int s = 0; s < 4 * MT; ++s) {
+    const int i = 4 * s + lk;
+    xr[s] = (i < K && ew < S.nt) ? (double)gather_slot<T>(S, ew, i, x, constrained) : 0.0;
+  }
+  pd4 acc[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) acc[t] = pd4{0, 0, 0, 0};
+  __syncthreads();
+  for (int q0 = 0; q0 < nq; q0 += RQ) {
+    double rp[PP], rc[PC];
+#pragma unroll
+    for (int k = 0; k < PP; ++k) {
+      const int s = tid + k * BLK, qq = s / NW, i = s - qq * NW, q = q0 + qq;
+      rp[k] = (q < nq && i < K) ? S.phi[(size_t)q * K + i] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < PC; ++k) {
+      const int s = tid + k * BLK, le = s / RQ, qq = s - le * RQ, q = q0 + qq;
+      rc[k] = (le < ne && q < nq) ? S.vpot[(size_t)(eb + le) * nq + q] * S.wq[q] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < PP; ++k) {
+      const int s = tid + k * BLK, qq = s / NW;
+      sP[qq][s - qq * NW] = rp[k];
+    }
+#pragma unroll
+    for (int k = 0; k < PC; ++k) {
+      const int s = tid + k * BLK, le = s / RQ;
+      sC[le][s - le * RQ] = rc[k] * sS[le];
+    }
+    __syncthreads();
+    pd4 u = pd4{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 4 * MT; ++s)
+      if (s < nks) u = __builtin_amdgcn_mfma_f64_16x16x4f64(sP[lr][4 * s + lk], xr[s], u, 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int qq = lk + 4 * j;
+      const double a = u[j] * sC[16 * wave + lr][qq];
+#pragma unroll
+      for (int t = 0; t < MT; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sP[qq][16 * t + lr], acc[t], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // value j of acc[t] in lane l: element 16 wave + l/16 + 4 j, mode 16 t + l%16
+#pragma unroll
+  for (int t = 0; t < MT; ++t)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int le = 16 * wave + lk + 4 * j, i = 16 * t + lr;
+      if (le < ne && i < K) yl[(size_t)(eb + le) * K + i] += (T)acc[t][j];
+    }
+}
+
+template <typename T, int MT>
+static void pot_gemm_mt(const DevSpace& S, double pot, const T* x, T* yl, int constrained, hipStream_t st) {
+  const int nb = (S.nt + POT_PE - 1) / POT_PE;
+  hipLaunchKernelGGL((k_pot_gemm<T, MT>), dim3(nb), dim3(BLK), 0, st, S, pot, x, yl, constrained);
+}
+
+// M_V x added to the element results yl: the matrix-core kernel up to 128
+// modes (order 7), the one-block-per-element kernel above that
+template <typename T>
+static void launch_potential(const DevSpace& S, double pot, const T* x, T* yl, int constrained, hipStream_t st) {
+  const int mt = (S.K + 15) / 16;
+  switch (mt) {
+    case 1: pot_gemm_mt<T, 1>(S, pot, x, yl, constrained, st); return;
+    case 2: pot_gemm_mt<T, 2>(S, pot, x, yl, constrained, st); return;
+    case 3: pot_gemm_mt<T, 3>(S, pot, x, yl, constrained, st); return;
+    case 4: pot_gemm_mt<T, 4>(S, pot, x, yl, constrained, st); return;
+    case 5: pot_gemm_mt<T, 5>(S, pot, x, yl, constrained, st); return;
+    case 6: pot_gemm_mt<T, 6>(S, pot, x, yl, constrained, st); return;
+    case 7: pot_gemm_mt<T, 7>(S, pot, x, yl, constrained, st); return;
+    case 8: pot_gemm_mt<T, 8>(S, pot, x, yl, constrained, st); return;
+    default: break;
+  }
+  size_t lds = (size_t)(S.K + S.nq) * sizeof(double);
+  if (lds > 64 * 1024) throw std::invalid_argument("quadrature too fine for the potential operator (qorder <= 20)");
+  hipLaunchKernelGGL(k_apply_potential<T>, dim3(S.nt), dim3(BLK), lds, st, S, pot, x, yl, constrained);
+}
+
 // ----------------------------------------------------------------- apply ----
 template <typename T> struct Fac;
 template <> struct Fac<double> {
@@ -617,9 +733,7 @@ static void apply_impl(const DevSpace& S, const OpCoef& op, const T* x, T* y, T*
   }
   if (op.pot != 0.0) {
     if (!S.vpot) throw std::logic_error("potential operator requested but no potential is set");
-    size_t lds = (size_t)(S.K + S.nq) * sizeof(double);
-    if (lds > 64 * 1024) throw std::invalid_argument("quadrature too fine for the potential operator (qorder <= 20)");
-    hipLaunchKernelGGL(k_apply_potential<T>, dim3(S.nt), dim3(BLK), lds, st, S, op.pot, x, yl, op.constrained);
+    launch_potential<T>(S, op.pot, x, yl, op.constrained, st);
   }
   int th = op.timed() ? timing_begin(kind + 2, st) : -1;
   if (slab) slab_scatter<T>(S, yl, y, x, op.constrained, partial, st, 1, 0, 0, op.cheb);
