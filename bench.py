@@ -217,6 +217,10 @@ def main():
     ap.add_argument("--no-eigen", action="store_true", help="skip the LOBPCG section (16^3, p = 6, 32 pairs)")
     ap.add_argument("--no-ks", action="store_true", help="skip the Kohn-Sham section (helium, 24^3, p = 5)")
     ap.add_argument("--serve", type=int, default=24, help="requests of the served-stream section (0 skips it)")
+    ap.add_argument("--aux-only", action="store_true",
+                    help="run only the LOBPCG and Kohn-Sham sections and print their records (for profiles); "
+                         "no bench line")
+    ap.add_argument("--ks-steps", type=int, default=20, help="SCF steps of the Kohn-Sham section")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="after the solves, copy a buffer of this many MiB three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -229,6 +233,16 @@ def main():
 
     def log(*a):
         print("[bench]", *a, file=sys.stderr, flush=True)
+
+    if args.aux_only:
+        ctx = api.Context(0)
+        aux = {}
+        if not args.no_eigen:
+            aux["eigen"] = eigen_section(ctx, log)
+        if not args.no_ks:
+            aux["kohn_sham"] = ks_section(ctx, log, steps=args.ks_steps)
+        print(json.dumps(aux, default=float), flush=True)
+        return
 
     t0 = time.perf_counter()
     xyz, tet = mesh.kuhn_cube(args.n, jitter=0.1, seed=2026)
@@ -434,7 +448,7 @@ def main():
         "cpu_baseline": None,
     }
     for key, skip, fn in (("eigen", args.no_eigen, lambda: eigen_section(ctx, log)),
-                          ("kohn_sham", args.no_ks, lambda: ks_section(ctx, log)),
+                          ("kohn_sham", args.no_ks, lambda: ks_section(ctx, log, steps=args.ks_steps)),
                           ("serving", args.serve <= 0, lambda: serve_section(log, count=args.serve))):
         if skip:
             continue
