@@ -175,6 +175,43 @@ def test_potential_operator_matches_assembly(pair):
     psp.set_potential(None)
 
 
+@pytest.mark.parametrize("M,q", [(1, 4), (2, 5), (5, 8), (7, 10), (8, 11), (2, 21)])
+def test_potential_operator_every_kernel(ctx, M, q):
+    """M_V against the oracle's assembly on each kernel width: the
+    matrix-core kernel at orders whose modes fill 1, 2, 4 and 8 tiles of 16
+    (p = 1, 2, 5, 7), at the Kohn-Sham rule q = p + 3 and at 9,261 points
+    (p = 2, q = 21; 125, 1,000 and 9,261 are not multiples of the 16-point
+    stage); the one-block-per-element kernel beyond 128 modes (p = 8)."""
+    import torch
+    _say("M_V kernel", M, q)
+    xyz, tet = mesh.kuhn_cube(2, jitter=0.1, seed=11)
+    osp = O.OracleSpace(M, xyz, tet, qorder=q)
+    psp = api.Space(ctx, M, xyz, tet, qorder=q)
+    try:
+        pts_ref, w = osp.quad3()
+        phi, _ = osp.basis_at(pts_ref)
+        qp = osp.qp_coords()
+        V = 2.0 + np.sin(qp[..., 0]) * qp[..., 1] - qp[..., 2] ** 2
+        vol = osp.volumes()
+        off, ind, val = osp.maps()
+        K = osp.K
+        x = np.random.default_rng(M).standard_normal(psp.ndof)
+        xl = osp.gather(x)
+        y = np.zeros(psp.ndof)
+        for e in range(osp.nt):
+            ye = vol[e] * (phi.T @ ((w * V[e]) * (phi @ xl[e])))
+            for i in range(K):
+                s = e * K + i
+                for t in range(off[s], off[s + 1]):
+                    y[ind[t]] += val[t] * ye[i]
+        psp.set_potential(torch.from_numpy(V).cuda())
+        got = psp.apply_op(torch.from_numpy(x).cuda(), stiff=0.0, potential=0.75).cpu().numpy()
+        assert np.abs(got - 0.75 * y).max() <= 1e-12 * np.abs(y).max()
+        psp.set_potential(None)
+    finally:
+        psp.close()
+
+
 def test_lda_perdew_zunger(pair):
     import torch
     _, psp = pair

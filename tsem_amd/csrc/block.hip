@@ -163,11 +163,13 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int n, int p, int q, const do
   }
 }
 
-// out[v] = sum_b partial[b nvals + v] in a fixed order: a block takes 32
-// values, lane group g (of 8) sums the partials b = g, g + 8, ... in
-// order, then the 8 group sums are added in group order.  Neighbouring
-// lanes read neighbouring values.
-constexpr int RV = 32, RG = 8;
+// out[v] = sum_b partial[b nvals + v] in a fixed order: a block takes 8
+// values, lane group g (of 32) sums the partials b = g, g + 32, ... in
+// order, then the 32 group sums are added in group order.  Neighbouring
+// lanes read neighbouring values.  (Groups of 8 over 32 values left 64
+// dependent adds per lane and few blocks: 18 us for 256 values of 512
+// partials.)
+constexpr int RV = 8, RG = 32;
 __global__ void __launch_bounds__(256) k_gram_reduce(int nvals, int nb, const double* __restrict__ partial,
                                                      double* __restrict__ out) {
   __shared__ double red[RG][RV];
