@@ -482,11 +482,13 @@ class Space:
                        converged=(rc == L.TSEM_OK))
 
 
-    def lobpcg(self, m, X=None, seed=7, tol=1e-8, criterion=1, max_iter=1000, precond=4, smooth_steps=3,
+    def lobpcg(self, m, X=None, seed=7, tol=1e-8, criterion=1, max_iter=1000, precond=4, smooth_steps=4,
                stiff=1.0, mass=0.0, potential=0.0, raise_on_noconv=True):
         """solve_LOBPCG: the m lowest eigenpairs of (stiff S + mass M + potential M_V) x
         = lambda M x on the eliminated operators (M_V: set_potential).  X: (m, n_dof) float64 device tensor
-        (row j = vector j), default N(0, 1) from `seed`.  Returns
+        (row j = vector j), default N(0, 1) from `seed`.  smooth_steps: the
+        Chebyshev degree of the preconditioning V-cycle (4: the helium SCF
+        runs 10% faster than with 3, the 32-pair Laplacian the same).  Returns
         (lambda (numpy, ascending), X, stats dict)."""
         import torch
         if X is None:

@@ -444,7 +444,7 @@ void tsem_server::eigen(Job& j, Entry& e) {
     lo.criterion = 1;
     lo.max_iter = 2000;
     lo.precond = 4;
-    lo.smooth_steps = 3;
+    lo.smooth_steps = 4;
     lo.stiff = 1.0;
     rc = tsem_lobpcg(sp, m, X, lam.data(), &lo, &ls);
     if (rc != TSEM_OK && rc != TSEM_ENOCONV) check(rc, "LOBPCG");
