@@ -248,8 +248,8 @@ def test_hydrogenic_spectrum(ctx):
         atom = KohnShamAtom(ctx, n=12, p=p, Z=2.0, occ=(0.0,) * 5, hartree=False, xc=False)
         sp = atom.sp
         sp.set_potential(atom.vext)
-        # the Coulomb cusp makes this pencil slow for LOBPCG at p = 6: about
-        # 500 iterations with a V-cycle (degree 3) as preconditioner
+        # the Coulomb cusp makes this pencil slow for LOBPCG at p = 6 (some
+        # hundreds of iterations with the V-cycle as preconditioner)
         lam, X, st = sp.lobpcg(5, tol=1e-9, stiff=0.5, potential=1.0, precond=4, max_iter=1000)
         assert st["converged"], st
         exact = np.array([-2.0, -0.5, -0.5, -0.5, -0.5])
