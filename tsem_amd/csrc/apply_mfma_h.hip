@@ -13,8 +13,10 @@
 //   the element's max |x| and split into hi/lo fp16 in LDS in the MFMA B
 //   layout [k / 8][element][8];
 //   U = D X (GEMM 1), V = Ghat U per element (FP32, kept in registers),
-//   exact per-element max |V| (LDS atomics), V / max split to fp16 hi/lo in
-//   LDS (element-major, the B-operand layout), Y = D^T V (GEMM 2), Y rescaled
+//   exact per-element max |V_a| of each derivative component (LDS atomics),
+//   Y = sum_a D_a^T V_a (GEMM 2) one component at a time: V_a / max split to
+//   fp16 hi/lo into one of two LDS tiles (element-major, the B-operand
+//   layout) and the accumulators rescaled between components; Y rescaled
 //   and written element-major through an LDS transpose, or, from the same
 //   LDS tile, as entity-slab entries (the element's contribution to each of
 //   its global DOFs, written where the slab scatter reads it).
@@ -67,17 +69,21 @@ __device__ inline void st8(uint16_t* p, h8 v) { *reinterpret_cast<h8*>(p) = v; }
 __device__ inline void st4(uint16_t* p, h4 v) { *reinterpret_cast<h4*>(p) = v; }
 }  // namespace
 
-// at least 3 waves per SIMD: the phases are latency bound, occupancy pays
-// more than registers (measured: 162 -> 128 us per launch on the 32^3, p = 6
-// mesh, no spills)
+// 4 waves per SIMD: the phases are latency bound, occupancy pays more than
+// registers.  The V tile holds one derivative component at a time (two
+// tiles, 37 KB per block at p = 6), so four blocks fit a CU's LDS; on the
+// 32^3, p = 6 mesh the kernel went 193 -> 185 us (p = 5: 151 -> 140 us) from
+// three blocks with all three components in one tile, and at two blocks it
+// took 234 us
 template <int CT>
-__global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, const float* __restrict__ x,
+__global__ void __launch_bounds__(NT, 4) k_elem_gemm_h(DevSpace S, OpCoef op, const float* __restrict__ x,
                                                     float* __restrict__ yl, int slab) {
   constexpr int NE = 16 * CT;
   constexpr int VS = NE + 1;  // odd row stride of the FP32 LDS tiles: no bank conflicts
   extern __shared__ unsigned char smem_raw[];
-  uint16_t* Vh = reinterpret_cast<uint16_t*>(smem_raw);              // hi [NE][R2], lo [NE][R2]  (phase C)
-  unsigned int* colmax = reinterpret_cast<unsigned int*>(smem_raw + S.hlds_v);  // [NE] float bits
+  // one derivative component of V at a time: hi [NE][R2], lo [NE][R2] (phase C)
+  uint16_t* Vh = reinterpret_cast<uint16_t*>(smem_raw);
+  unsigned int* colmax = reinterpret_cast<unsigned int*>(smem_raw + S.hlds_v);  // [3][NE] float bits
   x += (size_t)blockIdx.y * op.ldx;  // column of a block apply
   yl += (size_t)blockIdx.y * op.ldl;
   const int K = S.K, K32 = S.K32, Mp = S.Mp, R2 = S.R2;
@@ -97,7 +103,7 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
   const int e0 = xcd_block(blockIdx.x, gridDim.x) * NE;
   const int KB = K32 / 8;
   const float cst = (float)op.stiff * S.dinv;  // undo the factor scaling of GEMM 1
-  for (int s = tid; s < NE; s += NT) colmax[s] = 0u;
+  for (int s = tid; s < 3 * NE; s += NT) colmax[s] = 0u;
   // slab form: the block's entity offsets and orientation words are loaded
   // now, into registers, and staged into LDS only after GEMM 2, so that
   // their latency passes behind the GEMMs
@@ -285,7 +291,7 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
         float f = cst * __uint_as_float(smx[col]);
         g0 = f * g[0]; g1 = f * g[1]; g2 = f * g[2]; g3 = f * g[3]; g4 = f * g[4]; g5 = f * g[5];
       }
-      float vmax = 0.0f;
+      float vmax0 = 0.0f, vmax1 = 0.0f, vmax2 = 0.0f;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         int m = 16 * t + 4 * lk + j;
@@ -297,50 +303,27 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
         vv[tp][c][0][j] = v0;
         vv[tp][c][1][j] = v1;
         vv[tp][c][2][j] = v2;
-        vmax = fmaxf(vmax, fmaxf(fabsf(v0), fmaxf(fabsf(v1), fabsf(v2))));
+        vmax0 = fmaxf(vmax0, fabsf(v0));
+        vmax1 = fmaxf(vmax1, fabsf(v1));
+        vmax2 = fmaxf(vmax2, fabsf(v2));
       }
-      atomicMax(&colmax[col], __float_as_uint(vmax));
+      atomicMax(&colmax[col], __float_as_uint(vmax0));
+      atomicMax(&colmax[NE + col], __float_as_uint(vmax1));
+      atomicMax(&colmax[2 * NE + col], __float_as_uint(vmax2));
     }
   }
   __syncthreads();
 
-  // ---- phase B: V / max split to fp16 hi/lo straight from the registers,
-  // element-major with each derivative component padded to Mh (a multiple
-  // of 32) rows; rows from 16 Mt on are zero for GEMM 2
+  // ---- phase B: the tile rows from 16 Mt to Mh are zero for GEMM 2; no
+  // component writes them, so they are cleared once
   const int Mh = S.Mh;
-#pragma unroll
-  for (int tp = 0; tp < TPW; ++tp) {
-    const int t = wave + WAVES * tp;
-    if (t >= Mt) break;
-#pragma unroll
-    for (int c = 0; c < CT; ++c) {
-      const int col = 16 * c + lr;
-      const float m = __uint_as_float(colmax[col]);
-      const float inv = m > 0.0f ? 1.0f / m : 0.0f;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        h4 hi, lo;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const float sv = pin(vv[tp][c][a][j] * inv);  // one FP32 rounding for hi and lo
-          const _Float16 h = (_Float16)sv;
-          hi[j] = h;
-          lo[j] = (_Float16)(sv - (float)h);
-        }
-        const size_t o = (size_t)a * Mh + 16 * t + 4 * lk;
-        st4(Vh + (size_t)col * R2 + o, hi);
-        st4(Vh + (size_t)(NE + col) * R2 + o, lo);
-      }
-    }
-  }
   {
-    const int zr = (Mh - 16 * Mt) / 4;  // groups of 4 zero rows per component
-    for (int s = tid; s < 2 * NE * 3 * zr; s += NT) {
-      const int g = s % zr, r = s / zr, a = r % 3, row = r / 3;  // row: hi rows 0..NE-1, lo rows NE..2NE-1
-      st4(Vh + (size_t)row * R2 + (size_t)a * Mh + 16 * Mt + 4 * g, h4{0, 0, 0, 0});
+    const int zr = (Mh - 16 * Mt) / 4;  // groups of 4 zero rows
+    for (int s = tid; s < 4 * NE * zr; s += NT) {
+      const int g = s % zr, row = s / zr;  // rows of both buffers: hi, lo, hi, lo
+      st4(Vh + (size_t)row * R2 + 16 * Mt + 4 * g, h4{0, 0, 0, 0});
     }
   }
-  __syncthreads();
 
   // ---- phase C: GEMM 2, Y[i][e] = sum_r D[r][i] V[r][e].  Each wave owns
   // RJ row tiles and NC = 2 column tiles: with CT = 2 every column and the
@@ -368,9 +351,54 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
     kend[j] = (t2 < Rt) ? min(S.kend2h[t2], 16 * Mt) : 0;
     kmax = max(kmax, kend[j]);
   }
-  for (int a = 0; a < 3; ++a) {
+  // acc carries Y / run[c] for the lane's element column: each component is
+  // split with its own per-element maximum, and acc is brought to the new
+  // scale before the component is added (run stays 0 until a component with
+  // a nonzero maximum)
+  float run[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) run[c] = 0.0f;
+  // component a of V / its maxima, split to fp16 hi/lo from the registers,
+  // into tile buffer a & 1 (components 0 and 1 first, 2 once 0 is read)
+  auto put = [&](int a) {
+    uint16_t* T = Vh + (size_t)(a & 1) * 2 * NE * R2;
+#pragma unroll
+    for (int tp = 0; tp < TPW; ++tp) {
+      const int t = wave + WAVES * tp;
+      if (t >= Mt) break;
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        const int col = 16 * c + lr;
+        const float m = __uint_as_float(colmax[a * NE + col]);
+        const float inv = m > 0.0f ? 1.0f / m : 0.0f;
+        h4 hi, lo;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float sv = pin(vv[tp][c][a][j] * inv);  // one FP32 rounding for hi and lo
+          const _Float16 h = (_Float16)sv;
+          hi[j] = h;
+          lo[j] = (_Float16)(sv - (float)h);
+        }
+        const size_t o = (size_t)16 * t + 4 * lk;
+        st4(T + (size_t)col * R2 + o, hi);
+        st4(T + (size_t)(NE + col) * R2 + o, lo);
+      }
+    }
+  };
+  auto gemm = [&](int a) {
+    const uint16_t* T = Vh + (size_t)(a & 1) * 2 * NE * R2;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const float m = __uint_as_float(colmax[a * NE + 16 * ctile(c) + lr]);
+      if (m > 0.0f) {
+        const float f = run[c] / m;
+#pragma unroll
+        for (int j = 0; j < RJ; ++j) acc[j][c] *= f;
+        run[c] = m;
+      }
+    }
     for (int mm = 0; mm < kmax; mm += 32) {
-      const int rr = a * Mh + mm + 8 * lk;
+      const int rv = mm + 8 * lk, rr = a * Mh + rv;
       h8 ah[RJ], al[RJ], bh[NC], bl[NC];
 #pragma unroll
       for (int j = 0; j < RJ; ++j) {
@@ -383,8 +411,8 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         int col = 16 * ctile(c) + lr;
-        bh[c] = ld8(Vh + (size_t)col * R2 + rr);
-        bl[c] = ld8(Vh + (size_t)(NE + col) * R2 + rr);
+        bh[c] = ld8(T + (size_t)col * R2 + rv);
+        bl[c] = ld8(T + (size_t)(NE + col) * R2 + rv);
       }
 #pragma unroll
       for (int j = 0; j < RJ; ++j) {
@@ -398,7 +426,16 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
         }
       }
     }
-  }
+  };
+  put(0);
+  put(1);
+  __syncthreads();
+  gemm(0);
+  __syncthreads();  // buffer 0 read by every wave
+  put(2);
+  gemm(1);
+  __syncthreads();  // component 2 in place, buffer 1 read by every wave
+  gemm(2);
   __syncthreads();  // every wave is done reading Vh
 
   // ---- phase D: rescale and write Y element-major through an LDS transpose
@@ -431,7 +468,7 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int col = 16 * ctile(c) + lr;
-    const float m = __uint_as_float(colmax[col]) * S.dinv;
+    const float m = run[c] * S.dinv;
 #pragma unroll
     for (int j = 0; j < RJ; ++j) {
       int t2 = tile2(j);
@@ -566,17 +603,6 @@ __global__ void __launch_bounds__(NT, 3) k_elem_gemm_h(DevSpace S, OpCoef op, co
 
 int gemm_h_ct(const DevSpace& S) { return S.Mp <= 64 ? 4 : 2; }
 
-// LDS byte offset of the per-element maxima: after the largest of the FP32 V
-// tile, the fp16 V tile and the Y tile of one block
-size_t gemm_h_colmax_offset(const DevSpace& S) {
-  int NE = 16 * gemm_h_ct(S);
-  size_t vf = (size_t)3 * S.Mp * (NE + 1) * 4;
-  size_t vh = (size_t)2 * NE * S.R2 * 2;  // R2 = 3 Mh + 8
-  size_t y = (size_t)S.K * (NE + 1) * 4;
-  size_t m = vf > vh ? vf : vh;
-  return m > y ? m : y;
-}
-
 // end of the slab data staged behind the Y tile (slab_tab_offset): the
 // tables (slab_tab_bytes), then the epilogue's work-list counts, sign masks
 // and work lists (8 + NE * 10 ints)
@@ -594,13 +620,26 @@ bool slab_h_fits(const DevSpace& S) {
          (S.sl_n2 - 1) * VS < 65536;
 }
 
-size_t gemm_h_lds(const DevSpace& S) { return gemm_h_colmax_offset(S) + (size_t)16 * gemm_h_ct(S) * 4; }
+size_t gemm_h_lds(const DevSpace& S) { return gemm_h_colmax_offset(S) + (size_t)3 * 16 * gemm_h_ct(S) * 4; }
 
 // bytes of the gather phase's LDS (split tile, descriptors, masks and
 // maxima, face table rows, slot codes, sign masks), below the maxima of V
 static size_t gather_lds_end(const DevSpace& S) {
   const size_t NE = 16 * gemm_h_ct(S), R = 6 * (size_t)S.ndg2, J = S.ftc_J;
   return 4 * (size_t)S.K32 * NE + NE * sizeof(ElemDesc) + 8 * NE + R * J * 8 + R * 4 + (size_t)S.K * 4 + 24;
+}
+
+// LDS byte offset of the per-element maxima: after the largest of the
+// gather phase's data, one component's fp16 V tile, the Y tile and the slab
+// data behind it
+size_t gemm_h_colmax_offset(const DevSpace& S) {
+  const size_t NE = 16 * gemm_h_ct(S);
+  const size_t vh = (size_t)4 * NE * S.R2 * 2;  // two tiles, hi and lo
+  const size_t y = (size_t)S.K * (NE + 1) * 4;
+  size_t m = vh > y ? vh : y;
+  m = m > gather_lds_end(S) ? m : gather_lds_end(S);
+  if (S.sl_meta) m = m > slab_tab_end(S) ? m : slab_tab_end(S);
+  return (m + 15) & ~(size_t)15;
 }
 
 bool mfma_h_supported(const DevSpace& S) {

@@ -368,7 +368,7 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
     d.K32 = (K + 31) & ~31;
     d.Mh = (Kd + 31) & ~31;
     d.R2g = 3 * d.Mh;
-    d.R2 = 3 * d.Mh + 8;
+    d.R2 = d.Mh + 8;
     const int K16 = (K + 15) & ~15;
     double amax = 0;
     for (double v : Dp) amax = std::max(amax, std::fabs(v));
@@ -450,9 +450,6 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
   d.xt64 = sp->xt64.as<double>(); d.xt32 = sp->xt32.as<float>();
   d.Dh1hi = sp->Dh1hi.as<uint16_t>(); d.Dh1lo = sp->Dh1lo.as<uint16_t>();
   d.Dh2hi = sp->Dh2hi.as<uint16_t>(); d.Dh2lo = sp->Dh2lo.as<uint16_t>();
-  {
-    d.hlds_v = gemm_h_colmax_offset(d);
-  }
   if (sp->slab.ok) {
     const tsem::Slab& L = sp->slab;
     upload(sp->sl_meta, L.meta, st); upload(sp->sl_chunk, L.chunk, st); upload(sp->sl_eoff, L.eoff, st);
@@ -465,6 +462,7 @@ int tsem_space_create(tsem_ctx* ctx, int M, int nv, const double* xyz, int nt, c
     d.sl_eoff = sp->sl_eoff.as<int32_t>(); d.sl_perm = sp->sl_perm.as<int32_t>();
     d.sl_tk = sp->sl_tk.as<uint16_t>(); d.sl_tw = sp->sl_tw.as<double>(); d.sl_tw32 = sp->sl_tw32.as<float>();
   }
+  d.hlds_v = gemm_h_colmax_offset(d);  // after the slab fields: their tables share the region
   // assembled diagonal on the device from the scatter map, host copy for
   // the boundary-condition setup
   sp->diag_d.alloc((size_t)S.ndof * 8);

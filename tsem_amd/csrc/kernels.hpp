@@ -92,8 +92,9 @@ struct DevSpace {
   int kend2[16] = {0};
   // fp16-split FP32 operator: factors scaled by 1/dinv and split hi/lo,
   // Dh1[3][Mp][K32] (GEMM 1), Dh2[K16][3 Mh] (GEMM 2, D transposed, each
-  // component padded to Mh = roundup(Kd, 32) rows); element vectors
-  // (the element vectors are gathered and split inside the element kernel)
+  // component padded to Mh = roundup(Kd, 32) rows, row stride R2g = 3 Mh);
+  // the element kernel's LDS tile of one component of V has row stride
+  // R2 = Mh + 8 (the element vectors are gathered and split inside it)
   int K32 = 0, Mh = 0, R2 = 0, R2g = 0;
   size_t hlds_v = 0;  // LDS byte offset of the per-element maxima
   float dinv = 1.0f;
