@@ -362,9 +362,11 @@ __global__ void __launch_bounds__(NT) k_elem_gemm(DevSpace S, OpCoef op, const T
 // with X, then writes them.  The gather is that of k_gather_tiles (descriptors,
 // masks, face-table rows and slot codes staged in LDS, all staging loads
 // issued before the first wait); the GEMMs and the Y write are those of
-// k_elem_gemm.
+// k_elem_gemm.  Waves per SIMD: 3 in FP64 (194 registers before the bound),
+// 4 in FP32 (134 registers before the bound, 128 with a few spills: on the
+// 32^3 mesh 55.3 -> 47.6 us at p = 3 and 88.1 -> 77.7 us at p = 4 against 3).
 template <typename T, int CT>
-__global__ void __launch_bounds__(NT, 3) k_elem_gemm_gx(DevSpace S, OpCoef op, const T* __restrict__ Dp,
+__global__ void __launch_bounds__(NT, sizeof(T) == 4 ? 4 : 3) k_elem_gemm_gx(DevSpace S, OpCoef op, const T* __restrict__ Dp,
                                                      const T* __restrict__ x, T* __restrict__ yl) {
   using v4 = typename Mma<T>::v4;
   constexpr int NE = 16 * CT;
