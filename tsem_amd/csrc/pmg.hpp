@@ -58,7 +58,11 @@ class Pmg {
   std::vector<double> lmax;      // spectral radius estimate of D^-1 A_l
   std::vector<long> ndof_level;  // N_l
   int coarse_deg = 20;
-  double coarse_ratio = 30.0;  // the coarsest level's Chebyshev interval [hi / ratio, hi]
+  // the coarsest level's Chebyshev interval [hi / ratio, hi]: at degree 20,
+  // ratio 100 takes the 32^3, p = 6 solve from 101 to 98 iterations (mixed
+  // 170 -> 166 ms, FP64 344 -> 339 ms) and leaves the eigen solves as they
+  // were; 300 and degrees 12-28 were no better
+  double coarse_ratio = 100.0;
 
  private:
   tsem_space* fine;
