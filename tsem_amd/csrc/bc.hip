@@ -388,7 +388,11 @@ int tsem_dirichlet(tsem_space* sp, const double* uv, const double* ue, const dou
   op.stiff = 1.0; op.mass = 0.0; op.constrained = 1;
   launch_apply<double>(sp->d, op, vb.as<double>(), w.as<double>(), sp->yl64.as<double>(), nullptr, st);
   hipLaunchKernelGGL(k_dir_fix, dim3(nblk(n)), dim3(BLK), 0, st, n, d_free.as<uint8_t>(), w.as<double>(), b_dev);
-  for (int64_t i = 0; i < S.ndof; ++i) sp->cmask[i] = prev[i] | newmask[i];
+  bool grown = false;  // DOFs eliminated now that were free before
+  for (int64_t i = 0; i < S.ndof; ++i) {
+    grown |= newmask[i] && !prev[i];
+    sp->cmask[i] = prev[i] | newmask[i];
+  }
   upload_mask(sp, st);
   HIPCHK(hipGetLastError());
   if (bval) {
@@ -398,7 +402,11 @@ int tsem_dirichlet(tsem_space* sp, const double* uv, const double* ue, const dou
       if (!newmask[i]) bval[i] = NAN;
   }
   HIPCHK(hipStreamSynchronize(st));
-  sp->drop_pmg();
+  // the p-multigrid levels and the captured solver graphs depend on the
+  // elimination mask, not on the boundary values: with the mask as it was
+  // (new data on parts eliminated before, e.g. each Hartree solve of an SCF
+  // run) they stay valid
+  if (grown) sp->drop_pmg();
   API_END
 }
 

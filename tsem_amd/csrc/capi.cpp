@@ -748,15 +748,20 @@ int tsem_zero_bc(tsem_space* sp, int flag) {
   if (!sp) throw std::invalid_argument("null space");
   Space& S = sp->h;
   if (S.mark[0].empty()) { g_err = "boundary marks not set"; return TSEM_ESTATE; }
+  bool grown = false;  // DOFs eliminated now that were free before
   for (int dim = 0; dim <= 2; ++dim) {
     int ng = dim == 0 ? S.topo.nv : dim == 1 ? S.topo.ne : S.topo.nf;
     for (int g = 0; g < ng; ++g)
       if (S.mark[dim][g] == flag)
-        for (int i = 0; i < S.ndg[dim]; ++i) sp->cmask[S.geo_dof(dim, g) + i] = 1;
+        for (int i = 0; i < S.ndg[dim]; ++i) {
+          uint8_t& m = sp->cmask[S.geo_dof(dim, g) + i];
+          grown |= !m;
+          m = 1;
+        }
   }
   upload_mask(sp, sp->st());
   HIPCHK(hipStreamSynchronize(sp->st()));
-  sp->drop_pmg();
+  if (grown) sp->drop_pmg();  // the levels and graphs depend on the mask only
   API_END
 }
 
