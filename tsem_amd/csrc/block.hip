@@ -235,7 +235,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   typedef double v4 __attribute__((ext_vector_type(4)));
   constexpr int WPT = 4 / MT;  // waves per output tile
   constexpr int RTW = 4 / WPT; // 16-row tiles per wave and stage
-  __shared__ double sS[GP * CSP];
+  // the stage's 4 KS columns only (p <= 4 KS): 20 KB at KS = 8 instead of
+  // the 61 KB of the widest shape, so the LDS no longer holds every shape
+  // to 2 waves per SIMD
+  __shared__ double sS[4 * KS * CSP];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
   const int t = wave % MT, rw = wave / MT;
   const int pp = (p + 3) & ~3;
