@@ -129,8 +129,11 @@ __global__ void __launch_bounds__(BLK) k_apply_potential(DevSpace S, double pot,
 typedef double pd4 __attribute__((ext_vector_type(4)));
 constexpr int POT_PE = 64, POT_RQ = 16;
 
+// up to 4 mode tiles the kernel is held to 128 registers (4 waves per SIMD,
+// two spilled): at p = 5 the apply went 543 -> 525 us; wider tiles spill too
+// much under that bound (p = 6: 1,122 -> 1,262 us)
 template <typename T, int MT>
-__global__ void __launch_bounds__(BLK) k_pot_gemm(DevSpace S, double pot, const T* __restrict__ x,
+__global__ void __launch_bounds__(BLK, MT <= 4 ? 4 : 1) k_pot_gemm(DevSpace S, double pot, const T* __restrict__ x,
                                                   T* __restrict__ yl, int constrained) {
   constexpr int NW = 16 * MT, RQ = POT_RQ;
   constexpr int PP = RQ * NW / BLK, PC = POT_PE * RQ / BLK;  // stage loads per thread
