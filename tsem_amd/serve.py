@@ -483,7 +483,7 @@ class NativeServer:
             self.keep.clear()
 
 
-def stream(count=2000, load=0.8, slots=4, seed=2026, sat_count=300, log=print, rate=None):
+def stream(count=2000, load=0.8, slots=4, seed=2026, sat_count=300, log=print, rate=None, small_slots=-1):
     """The served stream of the C ABI server at one offered load, measured
     end to end: count Poisson requests (n uniform in 8..32, p in 3..8, seed =
     request id, jittered Kuhn meshes, manufactured problems evaluated in the
@@ -506,8 +506,9 @@ def stream(count=2000, load=0.8, slots=4, seed=2026, sat_count=300, log=print, r
         data[r.id] = (xyz, tet, M.cube_plane_marks(xyz))
         if (i + 1) % 500 == 0:
             log("stream: %d meshes (%.0f s)" % (i + 1, time.perf_counter() - t))
-    srv = NativeServer(slots=slots, small_cost=float(np.median([6 * n ** 3 * comb(p + 3, 3) for n in range(8, 33)
-                                                                  for p in range(3, 9)])))
+    srv = NativeServer(slots=slots, small_slots=small_slots,
+                       small_cost=float(np.median([6 * n ** 3 * comb(p + 3, 3) for n in range(8, 33)
+                                                   for p in range(3, 9)])))
 
     def submit(r):
         xyz, tet, lab = data[r.id]
@@ -575,7 +576,7 @@ def stream(count=2000, load=0.8, slots=4, seed=2026, sat_count=300, log=print, r
     worst = [dict(n=reqs[i].n, p=reqs[i].p, latency_s=float(lat[i]), isolated_s=iso[(reqs[i].n, reqs[i].p)],
                   queued_s=res[i]["queued_s"]) for i in np.argsort(ratio)[::-1][:5]]
     small = np.array([iso[(r.n, r.p)] <= np.median(list(iso.values())) for r in reqs])
-    out = dict(workload="served_poisson_n8-32_p3-8", requests=count, slots=slots, load=load,
+    out = dict(workload="served_poisson_n8-32_p3-8", requests=count, slots=slots, small_slots=small_slots, load=load,
                saturated_requests_per_s=sat_rate, offered_requests_per_s=rate, wall_s=wall, dofs_per_s=dofs / wall,
                p50_ms=float(np.percentile(lat, 50)) * 1e3, p99_ms=float(np.percentile(lat, 99)) * 1e3,
                latency_over_isolated={"p50": float(np.percentile(ratio, 50)), "p90": float(np.percentile(ratio, 90)),
@@ -599,8 +600,9 @@ def main():
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--sat", type=int, default=300, help="requests of the saturated-rate measurement")
     ap.add_argument("--rate", type=float, default=0.0, help="offered requests/s (default: load x saturated rate)")
+    ap.add_argument("--small-slots", type=int, default=-1, help="workers that take only small requests (-1: half)")
     a = ap.parse_args()
-    out = stream(a.requests, a.load, a.slots, sat_count=a.sat, rate=a.rate or None,
+    out = stream(a.requests, a.load, a.slots, sat_count=a.sat, rate=a.rate or None, small_slots=a.small_slots,
                  log=lambda s: print("[serve]", s, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
 
