@@ -270,3 +270,36 @@ def test_graphs_follow_pmg_rebuilds(ctx, mixed):
     xf, _ = fresh.pcg(bf, tol=1e-11, precond=4, mixed=mixed, check_every=7)
     assert np.linalg.norm(xf.cpu().numpy() - out[1][1]) <= 1e-9 * np.linalg.norm(xf.cpu().numpy())
     fresh.close()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_pmg_levels_kept_for_new_boundary_data(ctx, mixed):
+    """New Dirichlet data on the parts eliminated before leave the
+    elimination mask as it was, so the space keeps its p-multigrid levels
+    and captured graphs: a second problem solved on the same space after a
+    first one gives, with graphs on and off alike (bit for bit), the
+    solution of a fresh space."""
+    from tsem_amd import lib
+    _say("p-MG kept for new boundary data", mixed)
+    prob1, prob2 = problems.Manufactured(1234), problems.Manufactured(99)
+    xyz, tet = mesh.kuhn_cube(3, jitter=0.1, seed=13)
+    out = {}
+    try:
+        for g in (0, 1):
+            lib.check(lib.load().tsem_set_graphs(g))
+            psp = api.Space(ctx, 5, xyz, tet, qorder=8)
+            b1 = api.poisson_setup(psp, prob1)
+            psp.pcg(b1, tol=1e-11, precond=4, mixed=mixed, check_every=7)
+            b2 = api.poisson_setup(psp, prob2)
+            x2, s2 = psp.pcg(b2, tol=1e-11, precond=4, mixed=mixed, check_every=7)
+            out[g] = (x2.cpu().numpy(), s2["iterations"])
+            psp.close()
+    finally:
+        lib.check(lib.load().tsem_set_graphs(1))
+    assert out[0][1] == out[1][1] and np.array_equal(out[0][0], out[1][0])
+    fresh = api.Space(ctx, 5, xyz, tet, qorder=8)
+    bf = api.poisson_setup(fresh, prob2)
+    xf, _ = fresh.pcg(bf, tol=1e-11, precond=4, mixed=mixed, check_every=7)
+    xf = xf.cpu().numpy()
+    fresh.close()
+    assert np.linalg.norm(xf - out[1][0]) <= 1e-12 * np.linalg.norm(xf)
