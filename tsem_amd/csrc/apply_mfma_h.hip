@@ -325,7 +325,8 @@ __global__ void __launch_bounds__(NT, 4) k_elem_gemm_h(DevSpace S, OpCoef op, co
     }
   }
 
-  // ---- phase C: GEMM 2, Y[i][e] = sum_r D[r][i] V[r][e].  Each wave owns
+  // ---- phase C: GEMM 2, Y[i][e] = sum_a sum_r D_a[r][i] V_a[r][e], one
+  // derivative component a at a time from its LDS tile.  Each wave owns
   // RJ row tiles and NC = 2 column tiles: with CT = 2 every column and the
   // row tiles w + WAVES j; with CT = 4 (K <= 96, at most 6 row tiles) the
   // column tiles 2 (w & 1) + {0, 1} and the row tiles (w >> 1) + 2 j, so the
