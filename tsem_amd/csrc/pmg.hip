@@ -106,15 +106,26 @@ __global__ void k_csr_spmv(int n, const int32_t* __restrict__ rs, const int32_t*
 // one Chebyshev step on the assembled level, with its product fused in:
 // d = c1 d + c2 Dinv (b - A xin), xout = xin + d (xin and xout distinct: the
 // rows read their neighbours' xin)
+// one Chebyshev step with the CSR product fused in, CSR_LPR lanes per row:
+// lane q of a row's group sums the row's entries q, q + CSR_LPR, ..., and
+// the group adds its partial sums by a fixed xor-shuffle tree (the order is
+// fixed, so the step is reproducible).  A row per lane left half the SIMDs
+// idle on the 36k-row order-1 level of the 32^3 mesh.
+constexpr int CSR_LPR = 4;
 template <typename T>
 __global__ void k_cheb_csr(int n, const int32_t* __restrict__ rs, const int32_t* __restrict__ col,
                            const T* __restrict__ val, const T* __restrict__ b, const T* __restrict__ xin,
                            const T* __restrict__ inv, T c1, T c2, T* __restrict__ dv, T* __restrict__ xout) {
-  const int i = blockIdx.x * BLK + threadIdx.x;
-  if (i >= n) return;
+  const int gt = blockIdx.x * BLK + threadIdx.x;
+  const int i = gt / CSR_LPR, q = gt % CSR_LPR;
   const size_t o = (size_t)blockIdx.y * n;
   T acc = T(0);
-  for (int k = rs[i]; k < rs[i + 1]; ++k) acc += val[k] * xin[o + col[k]];
+  if (i < n)
+    for (int k = rs[i] + q; k < rs[i + 1]; k += CSR_LPR) acc += val[k] * xin[o + col[k]];
+  // every lane of the block reaches the shuffles (BLK is a multiple of CSR_LPR)
+#pragma unroll
+  for (int d = CSR_LPR / 2; d >= 1; d /= 2) acc += __shfl_xor(acc, d, CSR_LPR);
+  if (i >= n || q != 0) return;
   const T r = b[o + i] - acc;
   T dn = c2 * inv[i] * r;
   if (c1 != T(0)) dn += c1 * dv[o + i];
@@ -437,9 +448,10 @@ void Pmg::smooth_level(int l, T* x, const T* b, int degree, bool zero_init, hipS
     hipLaunchKernelGGL(k_cheb<T>, grid, dim3(BLK), 0, st, n, b, (const T*)nullptr, inv, T(0), T(1.0 / theta), dv,
                        buf[cur]);
     const T* val = sizeof(T) == 8 ? (const T*)csr_v64 : (const T*)csr_v32;
+    const dim3 cgrid(nblk(n * CSR_LPR), ncol);
     for (int k = 1; k < degree; ++k) {
       const double rho = 1.0 / (2 * sigma - rho_old);
-      hipLaunchKernelGGL(k_cheb_csr<T>, grid, dim3(BLK), 0, st, n, (const int32_t*)csr_rs, (const int32_t*)csr_col,
+      hipLaunchKernelGGL(k_cheb_csr<T>, cgrid, dim3(BLK), 0, st, n, (const int32_t*)csr_rs, (const int32_t*)csr_col,
                          val, b, (const T*)buf[cur], inv, T(rho * rho_old), T(2 * rho / delta), dv, buf[1 - cur]);
       cur = 1 - cur;
       rho_old = rho;
