@@ -8,10 +8,12 @@
 // the rounding of the parts leave a relative error of about 2^-22 per
 // product (FP32 itself: 2^-24), at 16x the FP32 MFMA rate per instruction.
 //
-// k_elem_gemm_h: per block of NE = 16 CT elements, the element coefficients
-//   gathered from x (element descriptors, face table rows in LDS), scaled by
-//   the element's max |x| and split into hi/lo fp16 in LDS in the MFMA B
-//   layout [k / 8][element][8];
+// k_elem_gemm_h: per block of NE = 16 CT elements, the entity DOFs of the
+//   elements loaded from x into an FP32 LDS tile (consecutive threads on
+//   consecutive DOFs of an entity), the element coefficients formed from it
+//   (element descriptors, face table rows in LDS), scaled by the element's
+//   max |x| and split into hi/lo fp16 in LDS in the MFMA B layout
+//   [k / 8][element][8];
 //   U = D X (GEMM 1), V = Ghat U per element (FP32, kept in registers),
 //   exact per-element max |V_a| of each derivative component (LDS atomics),
 //   Y = sum_a D_a^T V_a (GEMM 2) one component at a time: V_a / max split to
@@ -118,9 +120,10 @@ __global__ void __launch_bounds__(NT, 4) k_elem_gemm_h(DevSpace S, OpCoef op, co
     }
   }
   // ---- phase 0: the block's element coefficients, gathered from x through
-  // the element descriptors and the face table, scaled by each element's max
-  // |x| and split to fp16 hi/lo straight into LDS in the MFMA B layout
-  // [k / 8][element][8].  The region is the V tile's, dead until phase B.
+  // the element descriptors (phase 0a) and formed with the face table, scaled
+  // by each element's max |x| and split to fp16 hi/lo into LDS in the MFMA B
+  // layout [k / 8][element][8].  The region is the V tile's, dead until
+  // phase B.
   const int R = 6 * S.ndg2, J = S.ftc_J;
   uint16_t* xhh = reinterpret_cast<uint16_t*>(smem_raw);
   uint16_t* xhl = xhh + (size_t)KB * NE * 8;
@@ -132,6 +135,8 @@ __global__ void __launch_bounds__(NT, 4) k_elem_gemm_h(DevSpace S, OpCoef op, co
   int* cn = ck + (size_t)R * J;
   int* sslot = cn + R;
   uint32_t* ssgn = reinterpret_cast<uint32_t*>(sslot + K);
+  int* spos = reinterpret_cast<int*>(ssgn + 6);  // [K] entity order -> kind | geo << 2 | sub << 5
+  const int n1 = S.M - 1, n2 = S.ndg2, FB = 4 + 6 * n1, IB = FB + 4 * n2;
   const int nvalid = min(NE, S.nt - e0);
   const int dg = S.sl_diag;
   {
@@ -176,6 +181,59 @@ __global__ void __launch_bounds__(NT, 4) k_elem_gemm_h(DevSpace S, OpCoef op, co
     if (tid < K) sslot[tid] = slv;
     for (int s2 = tid + NT; s2 < K; s2 += NT) sslot[s2] = S.slot[s2];
     if (tid < 6) ssgn[tid] = sgv;
+    for (int p2 = tid; p2 < K; p2 += NT) {
+      int kind = 3, geo = 0, sub = p2 - IB;
+      if (p2 < 4) { kind = 0; geo = p2; sub = 0; }
+      else if (p2 < FB) { kind = 1; geo = (p2 - 4) / n1; sub = p2 - 4 - geo * n1; }
+      else if (p2 < IB) { kind = 2; geo = (p2 - FB) / n2; sub = p2 - FB - geo * n2; }
+      spos[p2] = kind | geo << 2 | sub << 5;
+    }
+  }
+  __syncthreads();
+  // ---- phase 0a: the block's entity DOFs, masked, into an FP32 LDS tile in
+  // entity order (4 vertices, 6 edges of n1, 4 faces of n2, the interior),
+  // [pos][element] with the element index swizzled by the row so that the
+  // writes here and the reads below are free of bank conflicts.  One DOF per
+  // thread and step, consecutive threads on consecutive DOFs of an element:
+  // the DOFs of an entity are contiguous in x, so a wave's loads touch a few
+  // lines per entity.  Loaded one element per lane, every lane touched a line
+  // of its own: the kernel took 184 us at p = 6 on the 32^3 mesh (173 us
+  // now), 350 at p = 7 (320) and 140 at p = 5 (141).  The tile lies in the
+  // split tile's region, dead until the split below.
+  float* rawt = reinterpret_cast<float*>(smem_raw);
+  {
+    constexpr int RQ = NE * (CT == 4 ? 96 : 192) / NT;
+    const int nk = nvalid * K;
+    const int dle = NT / K, dpos = NT - dle * K;
+    int le = tid / K, pos = tid - le * K;
+    int ad[RQ], ri[RQ];
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+      ad[q] = -1;
+      ri[q] = -1;
+      if (tid + NT * q < nk) {
+        const int code = spos[pos];
+        const int kind = code & 3, geo = (code >> 2) & 7, sub = code >> 5;
+        const ElemDesc& D = sd[le];
+        int a2, bit;
+        if (kind == 0) { a2 = D.v[geo]; bit = geo; }
+        else if (kind == 1) { a2 = (D.e[geo] & 0x7fffffff) + sub; bit = 4 + geo; }
+        else if (kind == 2) { a2 = D.f[geo] + sub; bit = 10 + geo; }
+        else { a2 = D.t + sub; bit = 14; }
+        ad[q] = ((sgm[le] >> bit) & 1) ? -1 : a2;
+        ri[q] = pos * NE + (le ^ (pos & (NE - 1)));
+      }
+      pos += dpos;
+      le += dle;
+      if (pos >= K) { pos -= K; ++le; }
+    }
+    // steps past the block's last DOF are skipped whole (a uniform branch)
+    float rv[RQ];
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) rv[q] = NT * q < nk ? x[ad[q] < 0 ? 0 : ad[q]] : 0.0f;
+#pragma unroll
+    for (int q = 0; q < RQ; ++q)
+      if (NT * q < nk && ri[q] >= 0) rawt[ri[q]] = ad[q] < 0 ? 0.0f : rv[q];
   }
   __syncthreads();
   // three groups of 8 modes per thread (NE * K32 / 8 = 768 groups: K32 <= 96
@@ -184,22 +242,19 @@ __global__ void __launch_bounds__(NT, 4) k_elem_gemm_h(DevSpace S, OpCoef op, co
   static_assert(NE * (CT == 4 ? 12 : 24) <= GQ * NT, "three groups of 8 modes per thread");
   const int KBl = K32l / 8;
   float xv[GQ][8];
+  auto rawv = [&](int le, int pos) -> float { return rawt[pos * NE + (le ^ (pos & (NE - 1)))]; };
   auto coef = [&](int le, int k) -> float {
-    const ElemDesc& D = sd[le];
-    const int msk = sgm[le];
     const int code = sslot[k];
     const int kind = code & 3, geo = (code >> 2) & 7, sub = code >> 5;
-    if (kind == 0) return ((msk >> geo) & 1) ? 0.0f : x[D.v[geo]];
+    if (kind == 0) return rawv(le, geo);
     if (kind == 1) {
-      const int ed = D.e[geo];
-      const float v = ((msk >> (4 + geo)) & 1) ? 0.0f : x[(ed & 0x7fffffff) + sub];
-      return (ed < 0 && (sub & 1)) ? -v : v;
+      const float v = rawv(le, 4 + geo * n1 + sub);
+      return (sd[le].e[geo] < 0 && (sub & 1)) ? -v : v;
     }
-    if (kind == 3) return ((msk >> 14) & 1) ? 0.0f : x[D.t + sub];
-    const int fo = D.fo[geo];
-    const bool fmask = (msk >> (10 + geo)) & 1;
+    if (kind == 3) return rawv(le, IB + sub);
+    const int fo = sd[le].fo[geo];
     if ((dg >> (fo & 7)) & 1) {
-      const float v = fmask ? 0.0f : x[D.f[geo] + sub];
+      const float v = rawv(le, FB + geo * n2 + sub);
       return (sub < 32 && ((ssgn[fo & 7] >> sub) & 1)) ? -v : v;
     }
     const int row = (fo & 7) * S.ndg2 + sub;
@@ -213,9 +268,9 @@ __global__ void __launch_bounds__(NT, 4) k_elem_gemm_h(DevSpace S, OpCoef op, co
       if (ie < 3) {
         const int le2 = (fo >> (3 + 3 * ie)) & 7;
         if (((fo >> (12 + ie)) & 1) && (idx & 1)) w = -w;
-        xq = ((msk >> (4 + le2)) & 1) ? 0.0f : x[(D.e[le2] & 0x7fffffff) + idx];
+        xq = rawv(le, 4 + le2 * n1 + idx);
       } else {
-        xq = fmask ? 0.0f : x[D.f[geo] + idx];
+        xq = rawv(le, FB + geo * n2 + idx);
       }
       acc += xq * w;
     }
@@ -623,11 +678,12 @@ bool slab_h_fits(const DevSpace& S) {
 
 size_t gemm_h_lds(const DevSpace& S) { return gemm_h_colmax_offset(S) + (size_t)3 * 16 * gemm_h_ct(S) * 4; }
 
-// bytes of the gather phase's LDS (split tile, descriptors, masks and
-// maxima, face table rows, slot codes, sign masks), below the maxima of V
+// bytes of the gather phase's LDS (split tile, which the entity tile of
+// phase 0a shares, descriptors, masks and maxima, face table rows, slot
+// codes, sign masks, entity codes), below the maxima of V
 static size_t gather_lds_end(const DevSpace& S) {
   const size_t NE = 16 * gemm_h_ct(S), R = 6 * (size_t)S.ndg2, J = S.ftc_J;
-  return 4 * (size_t)S.K32 * NE + NE * sizeof(ElemDesc) + 8 * NE + R * J * 8 + R * 4 + (size_t)S.K * 4 + 24;
+  return 4 * (size_t)S.K32 * NE + NE * sizeof(ElemDesc) + 8 * NE + R * J * 8 + R * 4 + (size_t)S.K * 8 + 24;
 }
 
 // LDS byte offset of the per-element maxima: after the largest of the
