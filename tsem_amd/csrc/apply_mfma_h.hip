@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(NT, 4) k_elem_gemm_h(DevSpace S, OpCoef op, co
   float xv[GQ][8];
   auto rawv = [&](int le, int pos) -> float { return rawt[pos * NE + (le ^ (pos & (NE - 1)))]; };
   auto coef = [&](int le, int k) -> float {
-    const int code = sslot[k];
+    const int code = CT == 4 ? S.slot[k] : sslot[k];
     const int kind = code & 3, geo = (code >> 2) & 7, sub = code >> 5;
     if (kind == 0) return rawv(le, geo);
     if (kind == 1) {
@@ -278,7 +278,11 @@ __global__ void __launch_bounds__(NT, 4) k_elem_gemm_h(DevSpace S, OpCoef op, co
   };
 #pragma unroll
   for (int g = 0; g < GQ; ++g) {
-    const int gi = tid + NT * g, kb = gi / NE, le = gi - kb * NE;
+    // with NE = 64 a wave holds one group of 8 modes over its 64 elements:
+    // the mode, its slot code (a scalar load) and its entity kind are uniform
+    // over the wave, and so are the branches on them (p = 6 on the 32^3
+    // mesh: 174 -> 169 us against per-lane codes from LDS; p = 5: 141 -> 137)
+    const int gi = tid + NT * g, kb = CT == 4 ? wave + (NT / NE) * g : gi / NE, le = CT == 4 ? lane : gi - kb * NE;
     float m = 0.0f;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
@@ -291,7 +295,7 @@ __global__ void __launch_bounds__(NT, 4) k_elem_gemm_h(DevSpace S, OpCoef op, co
   __syncthreads();
 #pragma unroll
   for (int g = 0; g < GQ; ++g) {
-    const int gi = tid + NT * g, kb = gi / NE, le = gi - kb * NE;
+    const int gi = tid + NT * g, kb = CT == 4 ? wave + (NT / NE) * g : gi / NE, le = CT == 4 ? lane : gi - kb * NE;
     if (kb < KBl) {
       const float m = __uint_as_float(smx[le]);
       h8 hi, lo;
