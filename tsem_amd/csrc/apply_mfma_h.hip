@@ -186,7 +186,10 @@ __global__ void __launch_bounds__(NT, 4) k_elem_gemm_h(DevSpace S, OpCoef op, co
       if (p2 < 4) { kind = 0; geo = p2; sub = 0; }
       else if (p2 < FB) { kind = 1; geo = (p2 - 4) / n1; sub = p2 - 4 - geo * n1; }
       else if (p2 < IB) { kind = 2; geo = (p2 - FB) / n2; sub = p2 - FB - geo * n2; }
-      spos[p2] = kind | geo << 2 | sub << 5;
+      // word of the entity's start in the element descriptor (v, e, f, then
+      // t after fo), its mask bit, the DOF's place in the entity
+      const int fw = kind == 0 ? geo : kind == 1 ? 4 + geo : kind == 2 ? 10 + geo : 18;
+      spos[p2] = fw | (kind == 3 ? 14 : fw) << 5 | sub << 9;
     }
   }
   __syncthreads();
@@ -197,11 +200,13 @@ __global__ void __launch_bounds__(NT, 4) k_elem_gemm_h(DevSpace S, OpCoef op, co
   // thread and step, consecutive threads on consecutive DOFs of an element:
   // the DOFs of an entity are contiguous in x, so a wave's loads touch a few
   // lines per entity.  Loaded one element per lane, every lane touched a line
-  // of its own: the kernel took 184 us at p = 6 on the 32^3 mesh (173 us
-  // now), 350 at p = 7 (320) and 140 at p = 5 (141).  The tile lies in the
-  // split tile's region, dead until the split below.
+  // of its own: the kernel took 184 us at p = 6 on the 32^3 mesh against
+  // 173 us with the tile, 350 at p = 7 (320) and 140 at p = 5 (141).  The
+  // tile lies in the split tile's region, dead until the split below.
   float* rawt = reinterpret_cast<float*>(smem_raw);
   {
+    constexpr int DW = sizeof(ElemDesc) / 4;
+    static_assert(offsetof(ElemDesc, t) == 72, "descriptor words: v 0-3, e 4-9, f 10-13, fo 14-17, t 18");
     constexpr int RQ = NE * (CT == 4 ? 96 : 192) / NT;
     const int nk = nvalid * K;
     const int dle = NT / K, dpos = NT - dle * K;
@@ -212,15 +217,13 @@ __global__ void __launch_bounds__(NT, 4) k_elem_gemm_h(DevSpace S, OpCoef op, co
       ad[q] = -1;
       ri[q] = -1;
       if (tid + NT * q < nk) {
+        // no branch on the entity kind, which differs across the lanes: the
+        // start is one descriptor word (the edge flag in bit 31 cleared).
+        // With a branch per kind, p = 6 on the 32^3 mesh took 169 us against
+        // 157 us without, p = 5 137 (130), p = 7 321 (306)
         const int code = spos[pos];
-        const int kind = code & 3, geo = (code >> 2) & 7, sub = code >> 5;
-        const ElemDesc& D = sd[le];
-        int a2, bit;
-        if (kind == 0) { a2 = D.v[geo]; bit = geo; }
-        else if (kind == 1) { a2 = (D.e[geo] & 0x7fffffff) + sub; bit = 4 + geo; }
-        else if (kind == 2) { a2 = D.f[geo] + sub; bit = 10 + geo; }
-        else { a2 = D.t + sub; bit = 14; }
-        ad[q] = ((sgm[le] >> bit) & 1) ? -1 : a2;
+        const int a2 = (reinterpret_cast<const int32_t*>(sd)[le * DW + (code & 31)] & 0x7fffffff) + (code >> 9);
+        ad[q] = ((sgm[le] >> ((code >> 5) & 15)) & 1) ? -1 : a2;
         ri[q] = pos * NE + (le ^ (pos & (NE - 1)));
       }
       pos += dpos;
