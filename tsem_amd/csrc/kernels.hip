@@ -688,9 +688,10 @@ void set_apply_path(int path) { g_apply_path = path; }
 int apply_path() { return g_apply_path; }
 bool split_path(const DevSpace& S, const OpCoef& op) {
   if (!(op.mass == 0.0 && g_apply_path == 0 && mfma_h_supported(S))) return false;
-  // measured on the 32^3 mesh, element kernel + scatter per apply: p = 2: 65
-  // (split) against 50 us (MFMA), p = 3: 86 / 76, p = 4: 122 / 117, p = 5:
-  // 176 / 182, p = 6: 234 / 319
+  // measured on the 32^3 mesh, element kernel + scatter per apply, split
+  // kernel against the MFMA kernel with its fused gather: p = 3: 85 / 66 us,
+  // p = 4: 118 / 106; from p = 5 on the split kernel is faster (p = 6: about
+  // 190 against 500 us)
   return op.pot != 0.0 || S.K > 35 || !mfma_gx_supported(S);
 }
 bool block_apply_supported(const DevSpace& S, const OpCoef& op) {
