@@ -690,8 +690,8 @@ bool split_path(const DevSpace& S, const OpCoef& op) {
   if (!(op.mass == 0.0 && g_apply_path == 0 && mfma_h_supported(S))) return false;
   // measured on the 32^3 mesh, element kernel + scatter per apply, split
   // kernel against the MFMA kernel with its fused gather: p = 3: 85 / 66 us,
-  // p = 4: 118 / 106; from p = 5 on the split kernel is faster (p = 6: about
-  // 190 against 500 us)
+  // p = 4: 118 / 106; from p = 5 on the split kernel is faster (p = 6: 234
+  // against 319 us with an earlier, slower split kernel)
   return op.pot != 0.0 || S.K > 35 || !mfma_gx_supported(S);
 }
 bool block_apply_supported(const DevSpace& S, const OpCoef& op) {
